@@ -1,0 +1,152 @@
+/* psx.h -- C ABI of libpsx.so, the gfx950 periodic-box P(k)/B(k) estimator.
+ *
+ * Each entry point replaces one stage of pySpectrum's periodic-box path; the
+ * comment above it cites the reference interface it stands in for
+ * (pyspectrum/pyspectrum.py and the f2py module built from
+ * pyspectrum/estimator.f).
+ *
+ * Conventions
+ *  - Every function returns an int status (PSX_OK = 0); on failure the
+ *    message is available from psx_last_error() (per calling thread).  No
+ *    C++ exception or abort crosses this boundary.
+ *  - Array arguments are DEVICE pointers unless marked [host].  The caller
+ *    owns them; the library owns only its context (plans, workspace).
+ *  - Grids are stored x fastest: element (x, y, z) of a full grid at
+ *    (z*N + y)*N + x; the half field (x <= N/2, the reference's
+ *    ifft_delta[:N//2+1]) at (z*N + y)*(N/2+1) + x.  Complex values are
+ *    interleaved (re, im) of float (prec F32/BF16X3) or double (prec F64).
+ *  - Work is enqueued on the context's stream (psx_ctx_set_stream); calls
+ *    return before the device finishes unless stated otherwise.
+ */
+#ifndef PSX_H
+#define PSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSX_OK 0
+#define PSX_ERR_ARG 1
+#define PSX_ERR_HIP 2
+#define PSX_ERR_NOMEM 3
+#define PSX_ERR_UNSUPPORTED 4
+
+/* Pipeline precision. */
+#define PSX_PREC_F64 0    /* FP64 FFTs, shells and MFMA contraction                */
+#define PSX_PREC_F32 1    /* FP32 FFTs and shells, FP32 MFMA, FP64 tile reduction  */
+#define PSX_PREC_BF16X3 2 /* FP32 FFTs and shells, split-BF16 3-term MFMA, FP32
+                             accumulation, FP64 tile reduction                     */
+
+typedef struct psx_ctx psx_ctx;
+
+/* ABI version of this header (bumped on any signature change). */
+int psx_abi_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* psx_last_error(void);
+
+/* Context: one per device and stream; plans and workspace are cached in it.
+ * (No reference counterpart: the reference keeps no state between calls.) */
+int psx_ctx_create(psx_ctx** ctx, int device);
+int psx_ctx_destroy(psx_ctx* ctx);
+int psx_ctx_set_stream(psx_ctx* ctx, void* hip_stream);
+int psx_ctx_synchronize(psx_ctx* ctx);
+/* Bytes of device workspace currently held by the context. */
+int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes);
+
+/* Replaces estimator.f:284-512 assign_quad (ia=ib=ic=id=0 branch), called at
+ * pyspectrum.py:951.  PCS assignment with interlacing of np particles.
+ * xyz: float32 [3][np] (x block, y block, z block); w: float32 [np] or NULL
+ * for unit weights; positions are clipped to [0, clip_max] (pyspectrum.py:
+ * 939-941).  grid_out: complex [N^3], re = grid A (the reference's odd
+ * slots), im = grid B (even slots).  wsum_abs = sum |w| sets the internal
+ * fixed-point scale (order-independent accumulation). */
+int psx_assign_quad(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                    float kf_ks, float clip_max, double wsum_abs, void* grid_out, int prec);
+
+/* Replaces the transform of _FFT (pyspectrum.py:1060-1080, pyfftw ifftn
+ * with normalise_idft=False): unnormalised 3D c2c FFT in place, exponent
+ * sign +1 (backward) or -1 (forward). */
+int psx_fft3d(psx_ctx* ctx, void* grid, int ngrid, int sign, int prec);
+
+/* Replaces estimator.f:605-675 fcomb_periodic (pyspectrum.py:957) followed by
+ * the half-field slice of pyspectrum.py:959: reads the transformed full
+ * grid, writes the half field.  sumw = sum of weights (cast to float32 as
+ * f2py does). */
+int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid, double sumw,
+                       int prec);
+
+/* Replaces FFT_periodic (pyspectrum.py:909-959): assign + FFT + fcomb. */
+int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                     double lbox, double sumw, double wsum_abs, void* half_out, int prec);
+
+/* Replaces reflect_delta (pyspectrum.py:1134-1157): half field -> full
+ * field written C-order [x][y][z] (z fastest), as the reference returns it. */
+int psx_reflect_delta(psx_ctx* ctx, const void* half, void* full_out, int ngrid, int prec);
+
+/* Entry of _Bk_periodic (pyspectrum.py:359) for an arbitrary full field:
+ * writes the half field of its Hermitian part (delta(k)+conj delta(-k))/2,
+ * whose shell fields equal the reference's Re(fftn(delta * mask)).
+ * full: complex of precision in_prec with element strides sx, sy, sz. */
+int psx_hermitian_half(psx_ctx* ctx, const void* full, int in_prec, int64_t sx, int64_t sy,
+                       int64_t sz, void* half_out, int ngrid, int prec);
+
+/* Replaces the binning loops of Pk_periodic (pyspectrum.py:691-714) and the
+ * shell power of _Bk_periodic (:404, in Parseval form): over the full field
+ * reflect_delta would build, bins b = int(|k|/step + 0.5) for b = 1..nbins;
+ * out3 = [3][nbins] float64 sums of (kf*|k|, |delta|^2, 1). */
+int psx_bin_power(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins, double kf,
+                  double* out3, int prec);
+
+/* Replaces estimator.f:155-264 pk_pbox_rsd (pyspectrum.py:634).  out: float64
+ * sums [5][nbin] (k, p0, p2, p4, nk) then [4][nbin][nmu] (km, mk, pkm, nkm),
+ * before the division by counts and kf^3 (done by the caller). */
+int psx_pk_pbox_rsd(psx_ctx* ctx, const void* half, int irsd, int lbox, int nbin, int nmu,
+                    int ngrid, double* out, int prec);
+
+/* Triangle list in the loop order of _Bk_periodic (pyspectrum.py:415-417). */
+int psx_bk_num_triangles(int step, int ncut, int nmax, int64_t* ntri /* [host] */);
+int psx_bk_triangles(int step, int ncut, int nmax, int32_t* ijl /* [host] [ntri][3] */);
+
+/* Replaces the shell loop of _Bk_periodic (pyspectrum.py:387-400):
+ * d_j(x) = Re FFT_forward[delta * 1(shell == j)] for j = Ncut//step..Nmax,
+ * written as real [ns][N^3] (float for F32/BF16X3, double for F64). */
+int psx_bk_shell_fields(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
+                        void* d_out, int prec);
+
+/* Replaces the triangle loop's einsum (pyspectrum.py:427-430):
+ * b_raw[t] = sum_x d_i d_j d_l for every triangle t, float64. */
+int psx_bk_contract(psx_ctx* ctx, const void* d, int ngrid, int step, int ncut, int nmax,
+                    double* b_raw, int prec);
+
+/* _Bk_periodic (pyspectrum.py:359-457) up to the normalisation: shell
+ * fields (workspace) + contraction; p0k[j-1] = sum_x d_j^2 / N^3 / Nk[j]
+ * ([nmax] float64, NaN for an empty shell, 0 below Ncut//step). */
+int psx_bk_raw(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
+               double* b_raw, double* p0k, int prec);
+
+/* Replaces _counts_Bk123 (pyspectrum.py:962-1030, cache-miss path):
+ * counts[t] = sum_x I_i I_j I_l of the unit shells in FP64 (= N^3 N_tri up to
+ * round-off), in triangle order. */
+int psx_bk_counts(psx_ctx* ctx, int ngrid, int step, int ncut, int nmax, double* counts);
+
+/* Replaces pyspectrum.py:418-445 and :346-355: normalisation by counts,
+ * unit conversion and shot noise, float64.  ijl: device int32 [ntri][3];
+ * a3 = (2 pi)^3, b3 = kf^3, a6 = (2 pi)^6, b6 = kf^6 as the caller computes
+ * them.  out7: [7][ntri] = p0k1, p0k2, p0k3, b123, q123, counts, b123_sn. */
+int psx_bk_finalize(psx_ctx* ctx, const double* b_raw, const double* p0k, const double* counts,
+                    const int32_t* ijl, int ntri, int ngrid, double nbar, double a3, double b3,
+                    double a6, double b6, double* out7);
+
+/* Replaces Bk_periodic (pyspectrum.py:285-356) end to end: positions ->
+ * out7 (see psx_bk_finalize) and p0k [nmax].  counts/ijl as above. */
+int psx_bk_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                    double lbox, double sumw, double wsum_abs, int step, int ncut, int nmax,
+                    const double* counts, const int32_t* ijl, double nbar, double a3, double b3,
+                    double a6, double b6, double* out7, double* p0k, int prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSX_H */

@@ -1,0 +1,33 @@
+"""Oracle for the periodic-box P(k)/B(k) estimator -- TEST INFRASTRUCTURE.
+
+This package is the checker, never the product.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The product path (``pyspectrum_amd``) never imports, links or runs
+anything here and fails loudly when its HIP library is missing.
+
+Contents
+--------
+``spectrum``  FP64 numpy restatement of the reference's periodic-box path
+              (``pyspectrum/pyspectrum.py`` and ``pyspectrum/estimator.f``);
+              every function cites the reference lines it follows.
+``ref``       ctypes binding to ``oracle/_ref/libestimator.so``, the reference's
+              own Fortran kernels (``assign_quad``, ``fcomb_periodic``,
+              ``pk_pbox_rsd``) compiled from ``/root/reference`` by
+              ``oracle/Makefile``.  Used only to pin ``spectrum`` in tests.
+
+Pins (how each oracle function is checked; see ``tests/test_oracle.py``):
+
+* ``assign_quad``       -- the reference Fortran (``ref``), mass/normalisation
+                           invariants, the 1-4-1 node stencil.
+* ``fft_packed``        -- textbook DFT (direct sum) on tiny grids.
+* ``fcomb_periodic``    -- the reference Fortran (``ref``) on random grids,
+                           closed form for one particle, delta(k=0) = 1.
+* ``reflect_delta``     -- Hermitian symmetry, brute-force index mapping.
+* ``shell_index``       -- the float formula at the reference lines, shell
+                           sizes 370/1402/3154 (step 3).
+* ``counts_Bk123``      -- the reference's FP64 count caches (committed
+                           fixtures), integer brute force on tiny grids.
+* ``bk_from_shells``    -- direct k-space triple sum on tiny grids.
+* ``pk_pbox_rsd``       -- the reference Fortran (``ref``), bit-exact bins.
+* ``Pk_periodic``       -- Parseval identity against the shell fields.
+"""

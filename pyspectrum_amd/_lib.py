@@ -1,0 +1,140 @@
+"""ctypes binding of libpsx.so (the gfx950 HIP library) and its status codes.
+
+The library is built in-tree by ``python -m pyspectrum_amd.build``.  There is
+no fallback: if the shared library is missing or no GPU is visible, every
+call raises.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsx.so")
+
+PSX_OK = 0
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+_c = ctypes
+_p = _c.c_void_p
+_i = _c.c_int
+_i64 = _c.c_int64
+_d = _c.c_double
+_f = _c.c_float
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "psx_abi_version": [],
+    "psx_ctx_create": [_c.POINTER(_p), _i],
+    "psx_ctx_destroy": [_p],
+    "psx_ctx_set_stream": [_p, _p],
+    "psx_ctx_synchronize": [_p],
+    "psx_ctx_workspace_bytes": [_p, _c.POINTER(_i64)],
+    "psx_assign_quad": [_p, _p, _p, _i64, _i, _f, _f, _d, _p, _i],
+    "psx_fft3d": [_p, _p, _i, _i, _i],
+    "psx_fcomb_periodic": [_p, _p, _p, _i, _d, _i],
+    "psx_fft_periodic": [_p, _p, _p, _i64, _i, _d, _d, _d, _p, _i],
+    "psx_reflect_delta": [_p, _p, _p, _i, _i],
+    "psx_hermitian_half": [_p, _p, _i, _i64, _i64, _i64, _p, _i, _i],
+    "psx_bin_power": [_p, _p, _i, _i, _i, _d, _p, _i],
+    "psx_pk_pbox_rsd": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "psx_bk_num_triangles": [_i, _i, _i, _c.POINTER(_i64)],
+    "psx_bk_triangles": [_i, _i, _i, _p],
+    "psx_bk_shell_fields": [_p, _p, _i, _i, _i, _i, _p, _i],
+    "psx_bk_contract": [_p, _p, _i, _i, _i, _i, _p, _i],
+    "psx_bk_raw": [_p, _p, _i, _i, _i, _i, _p, _p, _i],
+    "psx_bk_counts": [_p, _i, _i, _i, _i, _p],
+    "psx_bk_finalize": [_p, _p, _p, _p, _p, _i, _i, _d, _d, _d, _d, _d, _p],
+    "psx_bk_periodic": [_p, _p, _p, _i64, _i, _d, _d, _d, _i, _i, _i, _p, _p, _d, _d, _d, _d, _d,
+                        _p, _p, _i],
+}
+
+
+class PsxError(RuntimeError):
+    """A libpsx call returned a non-zero status."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load libpsx.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.isfile(LIB_PATH):
+                    raise PsxError("libpsx.so is not built (run `python -m pyspectrum_amd.build`)")
+                L = ctypes.CDLL(LIB_PATH)
+                for name, args in SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.argtypes = args
+                    fn.restype = _i
+                L.psx_last_error.argtypes = []
+                L.psx_last_error.restype = _c.c_char_p
+                _lib = L
+    return _lib
+
+
+def call(name, *args):
+    """Call ``name`` and raise PsxError with the library's message on failure."""
+    st = getattr(lib(), name)(*args)
+    if st != PSX_OK:
+        msg = lib().psx_last_error().decode(errors="replace")
+        raise PsxError("%s failed (status %d): %s" % (name, st, msg))
+    return st
+
+
+class Context:
+    """One library context on one device; work goes to torch's current stream."""
+
+    def __init__(self, device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise PsxError("no GPU visible: the HIP path has no CPU fallback")
+        self.device = device
+        self._h = _p()
+        with torch.cuda.device(device):
+            call("psx_ctx_create", ctypes.byref(self._h), device)
+
+    @property
+    def handle(self):
+        import torch
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        call("psx_ctx_set_stream", self._h, _p(stream))
+        return self._h
+
+    def workspace_bytes(self):
+        n = _i64()
+        call("psx_ctx_workspace_bytes", self._h, ctypes.byref(n))
+        return n.value
+
+    def close(self):
+        if self._h:
+            lib().psx_ctx_destroy(self._h)
+            self._h = _p()
+
+
+_contexts = {}
+
+
+def context(device=0):
+    ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = _contexts[device] = Context(device)
+    return ctx
+
+
+def ptr(t):
+    """Raw device pointer of a torch tensor (or None -> NULL)."""
+    return None if t is None else _p(t.data_ptr())
+
+
+def triangles(step, Ncut, Nmax):
+    """int32 [ntri, 3] triangle list in the reference's loop order (host)."""
+    import numpy as np
+    n = _i64()
+    call("psx_bk_num_triangles", step, Ncut, Nmax, ctypes.byref(n))
+    out = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+    call("psx_bk_triangles", step, Ncut, Nmax, out.ctypes.data_as(_p))
+    return out[: n.value]

@@ -1,0 +1,574 @@
+// C ABI, context, workspace and plan construction for libpsx.so.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "psx_ctx.h"
+
+namespace psx {
+
+// stage launchers from the other translation units
+int fixed_point_exponent(double wsum_abs);
+int launch_assign(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N, float kf_ks,
+                  float clip_max, double scale, int64_t* ga, int64_t* gb, int prec);
+int launch_fcomb(psx_ctx* ctx, const void* F, void* Hh, int N, double sumw, int prec,
+                 const void* rec, const float* Wf, const double* Wd);
+int launch_herm_half(psx_ctx* ctx, const void* F, int in_prec, int64_t sx, int64_t sy, int64_t sz,
+                     void* Hh, int N, int prec);
+int launch_reflect(psx_ctx* ctx, const void* Hh, void* F, int N, int prec);
+int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
+                     int prec, double* out3);
+int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int nbin, int nmu,
+                  int prec, double* gout);
+int launch_contract(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax, int prec,
+                    double* braw);
+int contract_maxpt(int prec);
+int launch_bk_finalize(psx_ctx* ctx, const double* braw, const double* p0k, const double* cnt,
+                       const int* tri, int ntri, double G, double nbar, double a3, double b3,
+                       double a6, double b6, double* out);
+int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k);
+
+std::string& last_error_string() {
+  static thread_local std::string s;
+  return s;
+}
+
+int set_error(int code, const std::string& msg) {
+  last_error_string() = msg;
+  return code;
+}
+
+static int prec_ok(int prec) {
+  return prec == PSX_PREC_F64 || prec == PSX_PREC_F32 || prec == PSX_PREC_BF16X3;
+}
+static size_t cbytes(int prec) { return prec == PSX_PREC_F64 ? 16 : 8; }
+static size_t rbytes(int prec) { return prec == PSX_PREC_F64 ? 8 : 4; }
+// The FFT/shell precision of a pipeline precision.
+static int fft_prec(int prec) { return prec == PSX_PREC_F64 ? PSX_PREC_F64 : PSX_PREC_F32; }
+
+static int check_ngrid(int N) {
+  FftShape sh;
+  if (N < 4 || N > 1024 || (N & 1) || !fft_factor(N, &sh))
+    return set_error(PSX_ERR_UNSUPPORTED,
+                     "Ngrid must be even, in [4, 1024] and a product of 2, 3 and 5");
+  return PSX_OK;
+}
+
+int ctx_buffer(psx_ctx* ctx, const char* name, size_t bytes, void** out) {
+  DevBuf& b = ctx->bufs[name];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      PSX_HIP(hipStreamSynchronize(ctx->stream));
+      PSX_HIP(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+      b.ptr = nullptr;
+      return set_error(PSX_ERR_NOMEM, std::string("workspace '") + name + "' of " +
+                                          std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+  }
+  *out = b.ptr;
+  return PSX_OK;
+}
+
+int ctx_alloc_owned(psx_ctx* ctx, size_t bytes, void** out) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes > 0 ? bytes : 16);
+  if (e != hipSuccess) return set_error(PSX_ERR_NOMEM, std::string("plan allocation: ") + hipGetErrorString(e));
+  ctx->owned.push_back(p);
+  *out = p;
+  return PSX_OK;
+}
+
+template <typename V>
+static int upload(psx_ctx* ctx, const std::vector<V>& h, V** d) {
+  void* p;
+  PSX_TRY(ctx_alloc_owned(ctx, h.size() * sizeof(V), &p));
+  if (!h.empty()) PSX_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
+  *d = (V*)p;
+  return PSX_OK;
+}
+
+// ------------------------------------------------------------- FFT plan ----
+// Twiddles exp(-2 pi i t/N) in long double, rounded once per precision.
+// fcomb tables follow estimator.f:607-645: tpi is a float32 parameter,
+// tpiL = tpi/float(N) a float32 quotient stored in double, recx =
+// cmplx(dcos(piL), dsin(piL)) a single-precision complex, xrec = recx^n by a
+// double recurrence; the window argument rk = tpiL*n is rounded to float32.
+int get_fft_plan(psx_ctx* ctx, int N, const FftPlan** out) {
+  auto it = ctx->fft_plans.find(N);
+  if (it != ctx->fft_plans.end()) { *out = &it->second; return PSX_OK; }
+  FftPlan p;
+  if (!fft_factor(N, &p.shape)) return set_error(PSX_ERR_UNSUPPORTED, "unsupported FFT length");
+  std::vector<c64> t64(N);
+  std::vector<c32> t32(N);
+  for (int t = 0; t < N; ++t) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)N;
+    t64[t] = mk((double)cosl(a), (double)sinl(a));
+    t32[t] = mk((float)t64[t].re, (float)t64[t].im);
+  }
+  PSX_TRY(upload(ctx, t64, (c64**)&p.tw64));
+  PSX_TRY(upload(ctx, t32, (c32**)&p.tw32));
+  const int Nh = N / 2 + 1;
+  const float tpi32 = 6.283185307f;
+  const double tpiL = (double)(float)(tpi32 / (float)N);
+  const double piL = -tpiL / 2.0;
+  const c64 rec1 = mk((double)(float)cos(piL), (double)(float)sin(piL));
+  std::vector<c64> rec(Nh);
+  c64 x = mk(1.0, 0.0);
+  for (int n = 0; n < Nh; ++n) { rec[n] = x; x = x * rec1; }
+  std::vector<float> Wf(Nh);
+  std::vector<double> Wd(Nh);
+  for (int n = 0; n < Nh; ++n) {
+    const float rk = (float)(tpiL * n);
+    if (rk != 0.0f) {
+      const float h = rk / 2.0f;
+      const float q = sinf(h) / h;
+      const float q2 = q * q;
+      Wf[n] = q2 * q2;
+      const double hd = (double)rk / 2.0;
+      const double qd = sin(hd) / hd;
+      Wd[n] = (qd * qd) * (qd * qd);
+    } else {
+      Wf[n] = 1.0f;
+      Wd[n] = 1.0;
+    }
+  }
+  PSX_TRY(upload(ctx, rec, (c64**)&p.rec));
+  PSX_TRY(upload(ctx, Wf, &p.Wf));
+  PSX_TRY(upload(ctx, Wd, &p.Wd));
+  ctx->fft_plans[N] = p;
+  *out = &ctx->fft_plans[N];
+  return PSX_OK;
+}
+
+// ----------------------------------------------------------- shell plan ----
+static int64_t shell_mmax(int j, int step) {
+  const int64_t a = (int64_t)(2 * j + 1) * step;
+  return (a * a - 1) / 4;  // largest m with shell_of_m(m, step) <= j
+}
+
+int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, const ShellPlan** out) {
+  auto key = std::make_tuple(N, step, ncut, nmax, kt);
+  auto it = ctx->shell_plans.find(key);
+  if (it != ctx->shell_plans.end()) { *out = it->second.get(); return PSX_OK; }
+  auto sp = std::make_unique<ShellPlan>();
+  sp->N = N; sp->step = step; sp->ncut = ncut; sp->nmax = nmax; sp->kt = kt;
+  sp->j0 = ncut / step;
+  sp->ns = nmax - sp->j0 + 1;
+  const int Nh = N / 2 + 1, hN = N / 2;
+  const int64_t mgrid = 3 * (int64_t)hN * hN;
+  std::vector<int> meta;
+  std::vector<Tile3> zt, yt, xt;
+  int64_t off = 0;
+  for (int s = 0; s < sp->ns; ++s) {
+    const int j = sp->j0 + s;
+    int64_t mm = std::min(shell_mmax(j, step), mgrid);
+    int rmax = (int)std::min<int64_t>(isqrt64(mm), hN);
+    const int xlim = std::min(rmax + 1, Nh);
+    const int kxw = (int)(cdiv(xlim, kt) * kt);
+    sp->mmax.push_back(mm);
+    sp->rmax.push_back(rmax);
+    sp->kxw.push_back(kxw);
+    sp->off.push_back(off);
+    meta.insert(meta.end(), {j, rmax, kxw, 0});
+    off += (int64_t)N * N * kxw;
+    for (int ky = 0; ky < N; ++ky) {
+      const int kys = ky <= hN ? ky : ky - N;
+      if (std::abs(kys) > rmax) continue;
+      for (int kx0 = 0; kx0 < xlim; kx0 += kt)
+        if ((int64_t)kx0 * kx0 + (int64_t)kys * kys <= mm) zt.push_back({s, ky, kx0, 0});
+    }
+    for (int z = 0; z < N; ++z)
+      for (int kx0 = 0; kx0 < xlim; kx0 += kt) yt.push_back({s, z, kx0, 0});
+    for (int z = 0; z < N; ++z)
+      for (int y0 = 0; y0 < N; y0 += 16) xt.push_back({s, z, y0, 0});
+  }
+  sp->t_elems = off;
+  sp->nz_tiles = (int)zt.size();
+  sp->ny_tiles = (int)yt.size();
+  sp->nx_tiles = (int)xt.size();
+  PSX_TRY(upload(ctx, zt, &sp->d_ztiles));
+  PSX_TRY(upload(ctx, yt, &sp->d_ytiles));
+  PSX_TRY(upload(ctx, xt, &sp->d_xtiles));
+  PSX_TRY(upload(ctx, meta, &sp->d_meta));
+  PSX_TRY(upload(ctx, sp->off, &sp->d_off));
+  PSX_TRY(upload(ctx, sp->mmax, &sp->d_mmax));
+  *out = sp.get();
+  ctx->shell_plans[key] = std::move(sp);
+  return PSX_OK;
+}
+
+// -------------------------------------------------------- triangle list ----
+static void triangle_list(int step, int ncut, int nmax, std::vector<int>* ijl) {
+  const int j0 = ncut / step;
+  ijl->clear();
+  for (int i = j0; i <= nmax; ++i)
+    for (int j = j0; j <= i; ++j)
+      for (int l = std::max(i - j, j0); l <= j; ++l) ijl->insert(ijl->end(), {i, j, l});
+}
+
+// ------------------------------------------------------- contract plan ----
+int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
+                      const ContractPlan** out);
+
+int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
+                      const ContractPlan** out) {
+  const int cprec = prec;
+  auto key = std::make_tuple(cprec, step, ncut, nmax);
+  auto it = ctx->contract_plans.find(key);
+  if (it != ctx->contract_plans.end()) { *out = it->second.get(); return PSX_OK; }
+  auto cp = std::make_unique<ContractPlan>();
+  const int j0 = ncut / step;
+  const int ns = nmax - j0 + 1;
+  cp->j0 = j0; cp->ns = ns; cp->nmax = nmax;
+  cp->nlt = (int)cdiv(ns, 16);
+  if (cp->nlt > 4)
+    return set_error(PSX_ERR_UNSUPPORTED, "at most 64 shells (Nmax - Ncut//step + 1 <= 64)");
+  // pairs (i, j), i >= j, i <= 2j, sorted by (j, i)
+  std::map<std::pair<int, int>, int> pidx;
+  std::vector<int> pair_s;
+  for (int j = j0; j <= nmax; ++j)
+    for (int i = j; i <= std::min(2 * j, nmax); ++i) {
+      pidx[{i, j}] = (int)pidx.size();
+      pair_s.push_back(i - j0);
+      pair_s.push_back(j - j0);
+    }
+  cp->npairs = (int)pidx.size();
+  cp->npt = (int)cdiv(cp->npairs, 16);
+  while ((int)pair_s.size() < cp->npt * 32) pair_s.push_back(-1);
+  std::vector<int> ijl;
+  triangle_list(step, ncut, nmax, &ijl);
+  cp->ntri = (int)(ijl.size() / 3);
+  std::vector<int> ptmask(cp->npt, 0);
+  std::vector<int> tpt(cp->ntri), trow(cp->ntri), tlt(cp->ntri), tcol(cp->ntri);
+  for (int t = 0; t < cp->ntri; ++t) {
+    const int i = ijl[3 * t], j = ijl[3 * t + 1], l = ijl[3 * t + 2];
+    const int p = pidx.at({i, j});
+    tpt[t] = p / 16; trow[t] = p % 16;
+    tlt[t] = (l - j0) / 16; tcol[t] = (l - j0) % 16;
+    ptmask[tpt[t]] |= 1 << tlt[t];
+  }
+  cp->nblk = 0;
+  for (int m : ptmask) cp->nblk += __builtin_popcount(m);
+  // assign pair tiles to waves: groups of 8 waves x maxpt slots, greedy by load
+  const int maxpt = contract_maxpt(prec);
+  const int WPG = 8;
+  const int ngroups = (int)cdiv(cp->npt, WPG * maxpt);
+  const int nw = ngroups * WPG;
+  std::vector<int> order(cp->npt);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return __builtin_popcount(ptmask[a]) > __builtin_popcount(ptmask[b]);
+  });
+  std::vector<int> load(nw, 0), cnt(nw, 0);
+  std::vector<int> wave_pt(nw * maxpt, -1), wave_mask(nw * maxpt, 0);
+  std::vector<int> pt_wave(cp->npt), pt_q(cp->npt);
+  for (int pt : order) {
+    int best = -1;
+    for (int w = 0; w < nw; ++w)
+      if (cnt[w] < maxpt && (best < 0 || load[w] < load[best])) best = w;
+    wave_pt[best * maxpt + cnt[best]] = pt;
+    wave_mask[best * maxpt + cnt[best]] = ptmask[pt];
+    pt_wave[pt] = best; pt_q[pt] = cnt[best];
+    cnt[best]++;
+    load[best] += __builtin_popcount(ptmask[pt]);
+  }
+  // output slots per group
+  std::vector<int> wave_slot(nw * maxpt * 4, -1);
+  std::vector<int> gslots(ngroups, 0);
+  for (int w = 0; w < nw; ++w)
+    for (int q = 0; q < maxpt; ++q)
+      for (int t = 0; t < 4; ++t)
+        if ((wave_mask[w * maxpt + q] >> t) & 1) wave_slot[(w * maxpt + q) * 4 + t] = gslots[w / WPG]++;
+  cp->nslot = *std::max_element(gslots.begin(), gslots.end());
+  std::vector<int> tri_loc(cp->ntri), tri_grp(cp->ntri);
+  for (int t = 0; t < cp->ntri; ++t) {
+    const int w = pt_wave[tpt[t]], q = pt_q[tpt[t]];
+    const int slot = wave_slot[(w * maxpt + q) * 4 + tlt[t]];
+    const int row = trow[t], col = tcol[t];
+    const int lane = col + 16 * (row / 4), reg = row % 4;
+    tri_loc[t] = slot * 256 + lane * 4 + reg;
+    tri_grp[t] = w / WPG;
+  }
+  cp->waves = nw;
+  cp->maxpt = maxpt;
+  PSX_TRY(upload(ctx, wave_pt, &cp->d_wave_pt));
+  PSX_TRY(upload(ctx, wave_mask, &cp->d_wave_mask));
+  PSX_TRY(upload(ctx, wave_slot, &cp->d_wave_slot));
+  PSX_TRY(upload(ctx, pair_s, &cp->d_pair_s));
+  PSX_TRY(upload(ctx, tri_loc, &cp->d_tri_loc));
+  PSX_TRY(upload(ctx, tri_grp, &cp->d_tri_grp));
+  PSX_TRY(upload(ctx, ijl, &cp->d_tri_ijl));
+  *out = cp.get();
+  ctx->contract_plans[key] = std::move(cp);
+  return PSX_OK;
+}
+
+// ------------------------------------------------------------ pipelines ----
+static int check_bk_args(int N, int step, int ncut, int nmax) {
+  PSX_TRY(check_ngrid(N));
+  PSX_REQUIRE(step >= 1 && ncut >= 0 && nmax >= 1, "step >= 1, Ncut >= 0, Nmax >= 1 required");
+  PSX_REQUIRE(ncut / step <= nmax, "Ncut//step must not exceed Nmax");
+  PSX_REQUIRE(nmax - ncut / step + 1 <= 64, "at most 64 shells (Nmax - Ncut//step + 1 <= 64)");
+  return PSX_OK;
+}
+
+static int fft_periodic_impl(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
+                             double lbox, double sumw, double wsum_abs, void* half, int prec) {
+  const int fp = fft_prec(prec);
+  const int64_t G = (int64_t)N * N * N;
+  void *ga, *gb, *grid;
+  PSX_TRY(ctx_buffer(ctx, "assign_a", G * 8, &ga));
+  PSX_TRY(ctx_buffer(ctx, "assign_b", G * 8, &gb));
+  PSX_TRY(ctx_buffer(ctx, "fft_grid", G * cbytes(fp), &grid));
+  const float kf_ks = (float)((double)N / lbox);           // np.float32(float(Ngrid)/Lbox)
+  const float clip_max = (float)(lbox * (1.0 - 1e-6));     // np.clip bound, cast to float32
+  const double scale = ldexp(1.0, fixed_point_exponent(wsum_abs));
+  PSX_TRY(launch_assign(ctx, xyz, w, np, N, kf_ks, clip_max, scale, (int64_t*)ga, (int64_t*)gb, fp));
+  PSX_TRY(launch_fft3d_from_fixed(ctx, (const int64_t*)ga, (const int64_t*)gb, 1.0 / scale, grid, N, fp));
+  const FftPlan* ft;
+  PSX_TRY(get_fft_plan(ctx, N, &ft));
+  PSX_TRY(launch_fcomb(ctx, grid, half, N, sumw, fp, ft->rec, ft->Wf, ft->Wd));
+  return PSX_OK;
+}
+
+static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                       double* braw, double* p0k, int prec) {
+  const int fp = fft_prec(prec);
+  const int j0 = ncut / step;
+  const int ns = nmax - j0 + 1;
+  const int64_t G = (int64_t)N * N * N;
+  void* D;
+  PSX_TRY(ctx_buffer(ctx, "shells", (size_t)ns * G * rbytes(fp), &D));
+  PSX_TRY(launch_shell_fields(ctx, half, N, step, ncut, nmax, fp, D));
+  PSX_TRY(launch_contract(ctx, D, N, step, ncut, nmax, prec, braw));
+  if (p0k) {
+    void* bins;
+    PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)3 * nmax * sizeof(double), &bins));
+    PSX_TRY(launch_bin_power(ctx, half, N, step, nmax, 1.0, fp, (double*)bins));
+    PSX_TRY(launch_p0k(ctx, (const double*)bins, nmax, j0, p0k));
+  }
+  return PSX_OK;
+}
+
+}  // namespace psx
+
+using namespace psx;
+
+// ================================================================= C ABI ===
+extern "C" {
+
+int psx_abi_version(void) { return 1; }
+
+const char* psx_last_error(void) { return last_error_string().c_str(); }
+
+int psx_ctx_create(psx_ctx** out, int device) {
+  PSX_REQUIRE(out != nullptr, "null ctx pointer");
+  int n = 0;
+  PSX_HIP(hipGetDeviceCount(&n));
+  PSX_REQUIRE(device >= 0 && device < n, "device index out of range");
+  PSX_HIP(hipSetDevice(device));
+  psx_ctx* c = new psx_ctx();
+  c->device = device;
+  *out = c;
+  return PSX_OK;
+}
+
+int psx_ctx_destroy(psx_ctx* ctx) {
+  if (!ctx) return PSX_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : ctx->bufs) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (void* p : ctx->owned) (void)hipFree(p);
+  delete ctx;
+  return PSX_OK;
+}
+
+int psx_ctx_set_stream(psx_ctx* ctx, void* stream) {
+  PSX_REQUIRE(ctx, "null ctx");
+  ctx->stream = (hipStream_t)stream;
+  return PSX_OK;
+}
+
+int psx_ctx_synchronize(psx_ctx* ctx) {
+  PSX_REQUIRE(ctx, "null ctx");
+  PSX_HIP(hipStreamSynchronize(ctx->stream));
+  return PSX_OK;
+}
+
+int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes) {
+  PSX_REQUIRE(ctx && bytes, "null argument");
+  int64_t s = 0;
+  for (auto& kv : ctx->bufs) s += (int64_t)kv.second.bytes;
+  *bytes = s;
+  return PSX_OK;
+}
+
+int psx_assign_quad(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                    float kf_ks, float clip_max, double wsum_abs, void* grid_out, int prec) {
+  PSX_REQUIRE(ctx && grid_out && (xyz || np == 0), "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(np >= 0, "negative particle count");
+  PSX_TRY(check_ngrid(ngrid));
+  const int64_t G = (int64_t)ngrid * ngrid * ngrid;
+  void *ga, *gb;
+  PSX_TRY(ctx_buffer(ctx, "assign_a", G * 8, &ga));
+  PSX_TRY(ctx_buffer(ctx, "assign_b", G * 8, &gb));
+  const double scale = ldexp(1.0, fixed_point_exponent(wsum_abs));
+  PSX_TRY(launch_assign(ctx, xyz, w, np, ngrid, kf_ks, clip_max, scale, (int64_t*)ga, (int64_t*)gb, fft_prec(prec)));
+  return launch_fixed_to_complex(ctx, (const int64_t*)ga, (const int64_t*)gb, 1.0 / scale, grid_out,
+                                 ngrid, fft_prec(prec));
+}
+
+int psx_fft3d(psx_ctx* ctx, void* grid, int ngrid, int sign, int prec) {
+  PSX_REQUIRE(ctx && grid, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(sign == 1 || sign == -1, "sign must be +1 or -1");
+  PSX_TRY(check_ngrid(ngrid));
+  return launch_fft3d(ctx, grid, ngrid, sign, fft_prec(prec));
+}
+
+int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid, double sumw,
+                       int prec) {
+  PSX_REQUIRE(ctx && grid && half_out, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_ngrid(ngrid));
+  const FftPlan* ft;
+  PSX_TRY(get_fft_plan(ctx, ngrid, &ft));
+  return launch_fcomb(ctx, grid, half_out, ngrid, sumw, fft_prec(prec), ft->rec, ft->Wf, ft->Wd);
+}
+
+int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                     double lbox, double sumw, double wsum_abs, void* half_out, int prec) {
+  PSX_REQUIRE(ctx && half_out && (xyz || np == 0), "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(np >= 0 && lbox > 0, "bad particle count or box size");
+  PSX_TRY(check_ngrid(ngrid));
+  return fft_periodic_impl(ctx, xyz, w, np, ngrid, lbox, sumw, wsum_abs, half_out, prec);
+}
+
+int psx_reflect_delta(psx_ctx* ctx, const void* half, void* full_out, int ngrid, int prec) {
+  PSX_REQUIRE(ctx && half && full_out, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_ngrid(ngrid));
+  return launch_reflect(ctx, half, full_out, ngrid, fft_prec(prec));
+}
+
+int psx_hermitian_half(psx_ctx* ctx, const void* full, int in_prec, int64_t sx, int64_t sy,
+                       int64_t sz, void* half_out, int ngrid, int prec) {
+  PSX_REQUIRE(ctx && full && half_out, "null argument");
+  PSX_REQUIRE(prec_ok(prec) && prec_ok(in_prec), "unknown precision");
+  PSX_TRY(check_ngrid(ngrid));
+  return launch_herm_half(ctx, full, fft_prec(in_prec), sx, sy, sz, half_out, ngrid, fft_prec(prec));
+}
+
+int psx_bin_power(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins, double kf,
+                  double* out3, int prec) {
+  PSX_REQUIRE(ctx && half && out3, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(step >= 1, "step >= 1 required");
+  PSX_TRY(check_ngrid(ngrid));
+  return launch_bin_power(ctx, half, ngrid, step, nbins, kf, fft_prec(prec), out3);
+}
+
+int psx_pk_pbox_rsd(psx_ctx* ctx, const void* half, int irsd, int lbox, int nbin, int nmu,
+                    int ngrid, double* out, int prec) {
+  PSX_REQUIRE(ctx && half && out, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_ngrid(ngrid));
+  return launch_pk_rsd(ctx, half, ngrid, irsd, lbox, nbin, nmu, fft_prec(prec), out);
+}
+
+int psx_bk_num_triangles(int step, int ncut, int nmax, int64_t* ntri) {
+  PSX_REQUIRE(ntri, "null argument");
+  PSX_REQUIRE(step >= 1 && ncut >= 0 && nmax >= 0, "bad step/Ncut/Nmax");
+  std::vector<int> ijl;
+  triangle_list(step, ncut, nmax, &ijl);
+  *ntri = (int64_t)(ijl.size() / 3);
+  return PSX_OK;
+}
+
+int psx_bk_triangles(int step, int ncut, int nmax, int32_t* out) {
+  PSX_REQUIRE(out, "null argument");
+  PSX_REQUIRE(step >= 1 && ncut >= 0 && nmax >= 0, "bad step/Ncut/Nmax");
+  std::vector<int> ijl;
+  triangle_list(step, ncut, nmax, &ijl);
+  std::memcpy(out, ijl.data(), ijl.size() * sizeof(int32_t));
+  return PSX_OK;
+}
+
+int psx_bk_shell_fields(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
+                        void* d_out, int prec) {
+  PSX_REQUIRE(ctx && half && d_out, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
+  return launch_shell_fields(ctx, half, ngrid, step, ncut, nmax, fft_prec(prec), d_out);
+}
+
+int psx_bk_contract(psx_ctx* ctx, const void* d, int ngrid, int step, int ncut, int nmax,
+                    double* b_raw, int prec) {
+  PSX_REQUIRE(ctx && d && b_raw, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
+  return launch_contract(ctx, d, ngrid, step, ncut, nmax, prec, b_raw);
+}
+
+int psx_bk_raw(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
+               double* b_raw, double* p0k, int prec) {
+  PSX_REQUIRE(ctx && half && b_raw, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
+  return bk_raw_impl(ctx, half, ngrid, step, ncut, nmax, b_raw, p0k, prec);
+}
+
+int psx_bk_counts(psx_ctx* ctx, int ngrid, int step, int ncut, int nmax, double* counts) {
+  PSX_REQUIRE(ctx && counts, "null argument");
+  PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
+  const int64_t Nh = ngrid / 2 + 1;
+  const int64_t nh = Nh * ngrid * ngrid;
+  void* ones;
+  PSX_TRY(ctx_buffer(ctx, "unit_half", nh * 16, &ones));
+  PSX_TRY(launch_fill_unit(ctx, (double*)ones, nh));
+  return bk_raw_impl(ctx, ones, ngrid, step, ncut, nmax, counts, nullptr, PSX_PREC_F64);
+}
+
+int psx_bk_finalize(psx_ctx* ctx, const double* b_raw, const double* p0k, const double* counts,
+                    const int32_t* ijl, int ntri, int ngrid, double nbar, double a3, double b3,
+                    double a6, double b6, double* out7) {
+  PSX_REQUIRE(ctx && b_raw && p0k && counts && ijl && out7, "null argument");
+  PSX_REQUIRE(ntri >= 0, "negative triangle count");
+  if (ntri == 0) return PSX_OK;
+  const double G = (double)ngrid * (double)ngrid * (double)ngrid;
+  return launch_bk_finalize(ctx, b_raw, p0k, counts, ijl, ntri, G, nbar, a3, b3, a6, b6, out7);
+}
+
+int psx_bk_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                    double lbox, double sumw, double wsum_abs, int step, int ncut, int nmax,
+                    const double* counts, const int32_t* ijl, double nbar, double a3, double b3,
+                    double a6, double b6, double* out7, double* p0k, int prec) {
+  PSX_REQUIRE(ctx && (xyz || np == 0) && counts && ijl && out7 && p0k, "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
+  const int fp = fft_prec(prec);
+  const int64_t nh = (int64_t)(ngrid / 2 + 1) * ngrid * ngrid;
+  void *half, *braw;
+  PSX_TRY(ctx_buffer(ctx, "bk_half", nh * cbytes(fp), &half));
+  std::vector<int> tl;
+  triangle_list(step, ncut, nmax, &tl);
+  const int ntri = (int)(tl.size() / 3);
+  PSX_TRY(ctx_buffer(ctx, "bk_braw", (size_t)std::max(ntri, 1) * sizeof(double), &braw));
+  PSX_TRY(fft_periodic_impl(ctx, xyz, w, np, ngrid, lbox, sumw, wsum_abs, half, prec));
+  PSX_TRY(bk_raw_impl(ctx, half, ngrid, step, ncut, nmax, (double*)braw, p0k, prec));
+  const double G = (double)ngrid * (double)ngrid * (double)ngrid;
+  if (ntri == 0) return PSX_OK;
+  return launch_bk_finalize(ctx, (const double*)braw, p0k, counts, ijl, ntri, G, nbar, a3, b3, a6,
+                            b6, out7);
+}
+
+}  // extern "C"

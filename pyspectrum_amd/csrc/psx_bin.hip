@@ -1,0 +1,205 @@
+// k-space binning of |delta|^2: P(k) shells (Pk_periodic), the shell powers
+// of the bispectrum (Parseval form of pyspectrum.py:404) and the P_l / P(k,mu)
+// estimator of pk_pbox_rsd (estimator.f:155-264).
+#include "psx_ctx.h"
+
+namespace psx {
+
+// ------------------------------------------------------------ bin power ---
+// Bins the full field that reflect_delta (pyspectrum.py:1134-1157) builds from
+// the half field: 1 <= x < N/2 count twice (the mirror has the same |delta|),
+// x = 0 and x = N/2 once, the seven Nyquist corners with Re(delta)^2.
+// bin = shell_of_m(|k|^2, step); bins 1..nbins are accumulated (k = kf*|k|).
+// Output sums (k, p, n) per bin; the caller divides.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin_power(const cplx<T>* __restrict__ Hh, int N, int step,
+                                                   int nbins, double kf, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* acc = reinterpret_cast<double*>(smem_raw);  // [3][nbins]
+  for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x) acc[i] = 0.0;
+  __syncthreads();
+  const int Nh = N / 2 + 1, hg = N / 2;
+  const int64_t n = (int64_t)Nh * N * N;
+  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(id % Nh);
+    const int y = (int)((id / Nh) % N);
+    const int z = (int)(id / ((int64_t)Nh * N));
+    const int ay = min(y, N - y), az = min(z, N - z);
+    const int64_t m = (int64_t)x * x + (int64_t)ay * ay + (int64_t)az * az;
+    const int b = shell_of_m(m, step);
+    if (b < 1 || b > nbins) continue;
+    const cplx<T> v = Hh[id];
+    const bool xe = (x == 0 || x == hg);
+    const bool corner = xe && (y == 0 || y == hg) && (z == 0 || z == hg);
+    const double re = (double)v.re, im = corner ? 0.0 : (double)v.im;
+    const double w = xe ? 1.0 : 2.0;
+    atomicAdd(&acc[b - 1], w * (kf * sqrt((double)m)));
+    atomicAdd(&acc[nbins + b - 1], w * (re * re + im * im));
+    atomicAdd(&acc[2 * nbins + b - 1], w);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x)
+    if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
+}
+
+__global__ void k_fill_f64(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
+                     int prec, double* out3) {
+  PSX_REQUIRE(nbins >= 1 && nbins <= 4096, "nbins out of range");
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, ctx->stream, out3, (int64_t)3 * nbins);
+  PSX_CHECK_LAUNCH();
+  const size_t lds = (size_t)3 * nbins * sizeof(double);
+  const int64_t n = (int64_t)(N / 2 + 1) * N * N;
+  const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), 2048);
+  if (prec == PSX_PREC_F64) {
+    if (lds > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_bin_power<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_bin_power<double>, dim3(blocks), dim3(256), lds, ctx->stream,
+                       (const cplx<double>*)half, N, step, nbins, kf, out3);
+  } else {
+    if (lds > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_bin_power<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_bin_power<float>, dim3(blocks), dim3(256), lds, ctx->stream,
+                       (const cplx<float>*)half, N, step, nbins, kf, out3);
+  }
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+// p0k[j-1] = sum |delta|^2 / Nk[j] for j = max(j0,1)..nmax (NaN if Nk = 0),
+// 0 elsewhere: the shell power of _Bk_periodic (pyspectrum.py:404) by Parseval.
+__global__ void k_p0k(const double* __restrict__ bins3, int nmax, int j0, double* __restrict__ p0k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmax) return;
+  const int j = i + 1;
+  p0k[i] = (j >= j0) ? bins3[nmax + i] / bins3[2 * nmax + i] : 0.0;
+}
+
+int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k) {
+  hipLaunchKernelGGL(k_p0k, dim3((unsigned)cdiv(nmax, 64)), dim3(64), 0, ctx->stream, bins3, nmax,
+                     j0, p0k);
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+// --------------------------------------------------------- pk_pbox_rsd ----
+// Literal restatement of the mode loop of estimator.f:196-244 over the full
+// grid: x <= N/2 read directly, x > N/2 from the mirrored element.  Every
+// float32 operation that decides a bin is replayed in the reference's order
+// without contraction (k bin: nint(Nbin*rk/(N/2)); mu bin:
+// int((|mu| + dmu)/dmu) in float64 of float32 values).
+struct RsdConst {
+  float kf, mubin, cth, sth, cph, sph;
+  int nbin, nmu;
+};
+
+__device__ __forceinline__ int fort_freq(int n, int N) { return n <= N / 2 ? n : n - N; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pk_rsd(const cplx<T>* __restrict__ Hh, int N, RsdConst c,
+                                                double* __restrict__ gout, int use_lds) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  // layout: [5][nbin] (k, p0, p2, p4, nk) then [4][nbin*nmu] (km, mk, pkm, nkm)
+  const int nb = c.nbin, nm = c.nmu;
+  const int ntot = 5 * nb + 4 * nb * nm;
+  double* acc = use_lds ? reinterpret_cast<double*>(smem_raw) : gout;
+  if (use_lds) {
+    for (int i = threadIdx.x; i < ntot; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+  }
+  const int Nh = N / 2 + 1;
+  const int64_t n = (int64_t)N * N * N;
+  const float fnb = (float)nb, fh = (float)(N / 2);
+  const double md = (double)c.mubin;
+  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (int64_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(id % N);
+    const int iy = (int)((id / N) % N);
+    const int iz = (int)(id / ((int64_t)N * N));
+    const float rkx = (float)fort_freq(ix, N), rky = (float)fort_freq(iy, N), rkz = (float)fort_freq(iz, N);
+    const float rk = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(rkx, rkx), __fmul_rn(rky, rky)), __fmul_rn(rkz, rkz)));
+    const int imk = (int)roundf(__fdiv_rn(__fmul_rn(fnb, rk), fh));
+    if (!(imk <= nb && imk != 0)) continue;
+    const float cot1 = __fdiv_rn(rkz, rk);
+    const float sit1 = __fsqrt_rn(__fsub_rn(1.0f, __fmul_rn(cot1, cot1)));
+    float cc = 0.0f;
+    if (sit1 > 0.0f) {
+      const float den = __fmul_rn(rk, sit1);
+      const float cp = __fdiv_rn(rkx, den), sp = __fdiv_rn(rky, den);
+      cc = __fadd_rn(__fmul_rn(c.sph, sp), __fmul_rn(c.cph, cp));
+    }
+    const float mu32 = __fadd_rn(__fmul_rn(c.cth, cot1), __fmul_rn(__fmul_rn(c.sth, sit1), cc));
+    const double mu = (double)mu32;
+    const int imu = (int)((fabs(mu) + md) / md);
+    const double mu2 = mu * mu;
+    const double Le2 = -0.5 + 1.5 * mu2;
+    const double Le4 = 0.375 - 3.75 * mu2 + 4.375 * (mu2 * mu2);
+    cplx<T> ct;
+    if (ix <= N / 2) ct = Hh[((int64_t)iz * N + iy) * Nh + ix];
+    else ct = Hh[((int64_t)((N - iz) % N) * N + (N - iy) % N) * Nh + (N - ix)];
+    const float a = hypotf((float)ct.re, (float)ct.im);
+    const double pk = (double)__fmul_rn(a, a);
+    const double kk = (double)__fmul_rn(c.kf, rk);
+    const int b = imk - 1;
+    atomicAdd(&acc[0 * nb + b], kk);
+    atomicAdd(&acc[1 * nb + b], pk);
+    atomicAdd(&acc[2 * nb + b], pk * 5.0 * Le2);
+    atomicAdd(&acc[3 * nb + b], pk * 9.0 * Le4);
+    atomicAdd(&acc[4 * nb + b], 1.0);
+    if (imu <= nm && imu > 0) {
+      const int bm = b * nm + (imu - 1);
+      double* a2 = acc + 5 * nb;
+      atomicAdd(&a2[0 * nb * nm + bm], kk);
+      atomicAdd(&a2[1 * nb * nm + bm], fabs(mu));
+      atomicAdd(&a2[2 * nb * nm + bm], pk);
+      atomicAdd(&a2[3 * nb * nm + bm], 1.0);
+    }
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < ntot; i += blockDim.x)
+      if (acc[i] != 0.0) atomicAdd(&gout[i], acc[i]);
+  }
+}
+
+int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int nbin, int nmu,
+                  int prec, double* gout) {
+  PSX_REQUIRE(irsd >= 0 && irsd <= 2, "rsd must be 0, 1 or 2");
+  PSX_REQUIRE(nbin >= 1 && nmu >= 1 && lbox > 0, "bad Nbin/Nmu/Lbox");
+  RsdConst c;
+  const float pi = 3.141592654f;
+  const float tpi = 2.0f * pi;
+  c.kf = tpi / (float)lbox;
+  c.mubin = 1.0f / (float)nmu;
+  const float th = irsd == 2 ? 0.0f : 0.5f * pi;
+  const float ph = irsd == 1 ? 0.5f * pi : 0.0f;
+  // sin/cos of these float32 constants, correctly rounded
+  c.cth = (float)cos((double)th); c.sth = (float)sin((double)th);
+  c.cph = (float)cos((double)ph); c.sph = (float)sin((double)ph);
+  c.nbin = nbin; c.nmu = nmu;
+  const int64_t ntot = 5 * (int64_t)nbin + 4 * (int64_t)nbin * nmu;
+  hipLaunchKernelGGL(k_fill_f64, dim3(64), dim3(256), 0, ctx->stream, gout, ntot);
+  PSX_CHECK_LAUNCH();
+  const size_t lds = (size_t)ntot * sizeof(double);
+  const int use_lds = lds <= 120 * 1024;
+  const int64_t n = (int64_t)N * N * N;
+  const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), use_lds ? 1024 : 4096);
+  const size_t shm = use_lds ? lds : 0;
+  if (prec == PSX_PREC_F64) {
+    if (shm > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_pk_rsd<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(k_pk_rsd<double>, dim3(blocks), dim3(256), shm, ctx->stream,
+                       (const cplx<double>*)half, N, c, gout, use_lds);
+  } else {
+    if (shm > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_pk_rsd<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(k_pk_rsd<float>, dim3(blocks), dim3(256), shm, ctx->stream,
+                       (const cplx<float>*)half, N, c, gout, use_lds);
+  }
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+}  // namespace psx

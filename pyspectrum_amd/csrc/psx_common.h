@@ -1,0 +1,82 @@
+// Shared device/host helpers for the periodic-box estimator library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "psx.h"
+
+namespace psx {
+
+// ---------------------------------------------------------------- complex --
+template <typename T> struct cplx { T re, im; };
+typedef cplx<float> c32;
+typedef cplx<double> c64;
+
+template <typename T> __host__ __device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> r; r.re = a; r.im = b; return r; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.re - b.re, a.im - b.im); }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(cplx<T> a, cplx<T> b) {
+  return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(cplx<T> a, T s) { return mk(a.re * s, a.im * s); }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> conj(cplx<T> a) { return mk(a.re, -a.im); }
+// multiply by +i (s > 0) or -i (s < 0)
+template <int S, typename T> __host__ __device__ __forceinline__ cplx<T> mul_i(cplx<T> a) {
+  return S > 0 ? mk(-a.im, a.re) : mk(a.im, -a.re);
+}
+
+// ------------------------------------------------------------ error state --
+struct Error {
+  int code = PSX_OK;
+  std::string msg;
+};
+
+std::string& last_error_string();
+int set_error(int code, const std::string& msg);
+
+#define PSX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return ::psx::set_error(PSX_ERR_HIP, std::string(#call) + ": " +             \
+                                               hipGetErrorString(_e));             \
+  } while (0)
+
+#define PSX_CHECK_LAUNCH()                                                         \
+  do {                                                                             \
+    hipError_t _e = hipGetLastError();                                             \
+    if (_e != hipSuccess)                                                          \
+      return ::psx::set_error(PSX_ERR_HIP, std::string("kernel launch: ") +        \
+                                               hipGetErrorString(_e));             \
+  } while (0)
+
+#define PSX_REQUIRE(cond, msg)                                                     \
+  do {                                                                             \
+    if (!(cond)) return ::psx::set_error(PSX_ERR_ARG, (msg));                      \
+  } while (0)
+
+#define PSX_TRY(call)                                                              \
+  do {                                                                             \
+    int _s = (call);                                                               \
+    if (_s != PSX_OK) return _s;                                                   \
+  } while (0)
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// exact floor(sqrt(x)) for x < 2^52
+__host__ __device__ __forceinline__ int64_t isqrt64(int64_t x) {
+  int64_t r = (int64_t)sqrt((double)x);
+  while (r * r > x) --r;
+  while ((r + 1) * (r + 1) <= x) ++r;
+  return r;
+}
+
+// Shell index of |k|^2 = m: int(sqrt(m)/step + 0.5) in exact integers
+// (largest n with (2n-1)*step <= 2*sqrt(m)); step = 1 gives the P(k) bins.
+__host__ __device__ __forceinline__ int shell_of_m(int64_t m, int step) {
+  int64_t q = isqrt64((4 * m) / ((int64_t)step * step));
+  return (int)((q + 1) / 2);
+}
+
+}  // namespace psx

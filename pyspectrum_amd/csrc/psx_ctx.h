@@ -1,0 +1,101 @@
+// Library context: device, stream, cached plans and a grow-only workspace.
+#pragma once
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "psx_common.h"
+#include "psx_fft.h"
+
+struct psx_ctx;
+
+namespace psx {
+
+// Device buffer owned by the context (freed at ctx destroy).
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Per-N twiddle table, both precisions.
+struct FftPlan {
+  FftShape shape;
+  void* tw32 = nullptr;  // cplx<float>[N]
+  void* tw64 = nullptr;  // cplx<double>[N]
+  // fcomb tables, n = 0..N/2: twiddle recurrence (cplx<double>) and PCS window
+  void* rec = nullptr;
+  float* Wf = nullptr;
+  double* Wd = nullptr;
+};
+
+// One pruned-pass tile: (shell slot, a, b) with meaning per pass.
+struct Tile3 {
+  int s, a, b, pad;
+};
+
+// Shell-field plan for (Ngrid, step, Ncut, Nmax).
+struct ShellPlan {
+  int N = 0, step = 0, ncut = 0, nmax = 0, kt = 16;  // kt: kx tile width of the z/y passes
+  int j0 = 0, ns = 0;                   // shells j0..nmax, ns of them
+  std::vector<int64_t> mmax;            // per slot: max |k|^2 of the shell
+  std::vector<int> rmax;                // per slot: max |k component|
+  std::vector<int> kxw;                 // per slot: stored kx width (multiple of 16)
+  std::vector<int64_t> off;             // per slot: element offset in the T buffers
+  int64_t t_elems = 0;                  // total elements of T1 (= T2)
+  int nz_tiles = 0, ny_tiles = 0, nx_tiles = 0;
+  Tile3* d_ztiles = nullptr;
+  Tile3* d_ytiles = nullptr;
+  Tile3* d_xtiles = nullptr;
+  int* d_meta = nullptr;                // per slot: j, rmax, kxw, (int)mmax -- device copy
+  int64_t* d_off = nullptr;
+  int64_t* d_mmax = nullptr;
+};
+
+// MFMA contraction plan (pairs x shells, 16x16 blocks restricted to triangles).
+struct ContractPlan {
+  int ns = 0, j0 = 0, nmax = 0;
+  int npairs = 0, npt = 0, nlt = 0, nblk = 0, ntri = 0;
+  int waves = 0, maxpt = 0;             // total waves (groups x waves/WG), pair tiles per wave
+  int nslot = 0;                        // valid output blocks per group (max over groups)
+  int* d_wave_pt = nullptr;             // [waves][maxpt] pair tile (-1 = none)
+  int* d_wave_mask = nullptr;           // [waves][maxpt] bitmask of l tiles
+  int* d_wave_slot = nullptr;           // [waves][maxpt][4] output slot of each block (-1 = none)
+  int* d_pair_s = nullptr;              // [npt*16][2] shell slots (i-j0, j-j0) per row (-1 pad)
+  int* d_tri_loc = nullptr;             // [ntri] slot*256 + lane*4 + reg
+  int* d_tri_grp = nullptr;             // [ntri] group of the triangle's pair tile
+  int* d_tri_ijl = nullptr;             // [ntri][3] shell indices i, j, l (reference loop order)
+};
+
+}  // namespace psx
+
+struct psx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<int, psx::FftPlan> fft_plans;
+  std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<psx::ShellPlan>> shell_plans;
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<psx::ContractPlan>> contract_plans;
+  std::map<std::string, psx::DevBuf> bufs;
+  std::vector<void*> owned;  // plan allocations
+};
+
+namespace psx {
+// Grow-only named workspace buffer.
+int ctx_buffer(psx_ctx* ctx, const char* name, size_t bytes, void** out);
+int ctx_alloc_owned(psx_ctx* ctx, size_t bytes, void** out);
+int get_fft_plan(psx_ctx* ctx, int N, const FftPlan** out);
+int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, const ShellPlan** out);
+int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
+                      const ContractPlan** out);
+
+// stage launchers (defined in the .hip files)
+int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec);
+int launch_fft3d_from_fixed(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
+                            void* grid, int N, int prec);
+int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                        int prec, void* dout);
+int launch_fixed_to_complex(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
+                            void* grid, int N, int prec);
+int launch_fill_unit(psx_ctx* ctx, double* half, int64_t n);
+}  // namespace psx

@@ -1,0 +1,319 @@
+// 3D FFTs of the periodic-box estimator on gfx950.
+//
+//  * Data transform (pyspectrum.py:1060-1080): the packed interlaced grid
+//    A + iB, backward (e^{+i}), unnormalised, as three batched line passes
+//    (x, y, z).  The x pass converts the fixed-point assignment grids on load.
+//  * Shell fields (pyspectrum.py:387-400): d_j(x) = Re FFT_forward[delta 1_j]
+//    for every shell j, as a pruned Hermitian (c2r) transform of the half
+//    field: a z pass over the (kx, ky) lines that meet the shell, a y pass
+//    over kx <= kmax_j columns, and an x pass that builds two Hermitian lines
+//    per complex FFT (their real and imaginary parts are the two real outputs).
+//
+// Grid layout everywhere: index (z*N + y)*N + x, x fastest; the half field is
+// (z*N + y)*(N/2+1) + x.
+#include "psx_ctx.h"
+
+namespace psx {
+
+// ------------------------------------------------------------------ passes --
+template <typename T> struct TileCfg;
+template <> struct TileCfg<float> { static constexpr int ELEMS = 8192; static constexpr int LINES = 16; };
+template <> struct TileCfg<double> { static constexpr int ELEMS = 4096; static constexpr int LINES = 8; };
+
+template <typename T>
+__device__ __forceinline__ int tile_lines(int N) {
+  int tl = TileCfg<T>::ELEMS / N;
+  return tl < TileCfg<T>::LINES ? tl : TileCfg<T>::LINES;
+}
+
+// Batched line FFT.  Line (o, c) starts at o*Os + c*Cs, element n at +n*Es.
+// A tile holds tl consecutive inner lines c of one outer index o.
+// FIXED: the input is a pair of int64 fixed-point grids (re, im) scaled by inv_scale.
+template <typename T, int S, bool FIXED>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
+    const cplx<T>* in, cplx<T>* out, const int64_t* __restrict__ ga, const int64_t* __restrict__ gb,
+    double inv_scale, FftShape sh, const cplx<T>* __restrict__ tw, int64_t Os, int64_t Cs,
+    int64_t Es, int n_inner) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int N = sh.N;
+  const int lstride = N + 1;
+  const int tl = tile_lines<T>(N);
+  const int ntile = (n_inner + tl - 1) / tl;
+  const int64_t o = blockIdx.x / ntile;
+  const int c0 = (int)(blockIdx.x - o * ntile) * tl;
+  const int nl = min(tl, n_inner - c0);
+  const int64_t base = o * Os + (int64_t)c0 * Cs;
+  const int total = nl * N;
+  for (int idx = threadIdx.x; idx < total; idx += FFT_THREADS) {
+    int c, n;
+    if (Es == 1) { c = idx / N; n = idx - c * N; }
+    else { n = idx / nl; c = idx - n * nl; }
+    const int64_t g = base + c * Cs + n * Es;
+    cplx<T> v;
+    if constexpr (FIXED) {
+      v = mk((T)((double)ga[g] * inv_scale), (T)((double)gb[g] * inv_scale));
+    } else {
+      v = in[g];
+    }
+    lds[c * lstride + n] = v;
+  }
+  __syncthreads();
+  fft_tile<S, T, TileCfg<T>::ELEMS>(lds, sh, nl, lstride, tw);
+  for (int idx = threadIdx.x; idx < total; idx += FFT_THREADS) {
+    int c, n;
+    if (Es == 1) { c = idx / N; n = idx - c * N; }
+    else { n = idx / nl; c = idx - n * nl; }
+    out[base + c * Cs + n * Es] = lds[c * lstride + n];
+  }
+}
+
+template <typename T>
+static size_t lds_bytes(int N) {
+  int tl = TileCfg<T>::ELEMS / N;
+  if (tl > TileCfg<T>::LINES) tl = TileCfg<T>::LINES;
+  return (size_t)tl * (N + 1) * sizeof(cplx<T>);
+}
+
+template <typename T> static const cplx<T>* tw_of(const FftPlan* p);
+template <> const cplx<float>* tw_of<float>(const FftPlan* p) { return (const cplx<float>*)p->tw32; }
+template <> const cplx<double>* tw_of<double>(const FftPlan* p) { return (const cplx<double>*)p->tw64; }
+
+template <typename K>
+static int set_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) {
+    PSX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+  }
+  return PSX_OK;
+}
+
+template <typename T, int S, bool FIXED>
+static int run_pass(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out,
+                    const int64_t* ga, const int64_t* gb, double inv_scale, int axis) {
+  const int N = p->shape.N;
+  const int64_t NN = (int64_t)N * N;
+  int64_t Os, Cs, Es;
+  if (axis == 0) { Os = NN; Cs = N; Es = 1; }        // x lines: inner y, outer z
+  else if (axis == 1) { Os = NN; Cs = 1; Es = N; }   // y lines: inner x, outer z
+  else { Os = N; Cs = 1; Es = NN; }                  // z lines: inner x, outer y
+  int tl = TileCfg<T>::ELEMS / N;
+  if (tl > TileCfg<T>::LINES) tl = TileCfg<T>::LINES;
+  const int64_t nblocks = (int64_t)N * cdiv(N, tl);
+  const size_t lds = lds_bytes<T>(N);
+  auto kern = k_fft_lines<T, S, FIXED>;
+  PSX_TRY(set_lds(kern, lds));
+  hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(FFT_THREADS), lds, ctx->stream, in, out,
+                     ga, gb, inv_scale, p->shape, tw_of<T>(p), Os, Cs, Es, N);
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+template <typename T, int S>
+static int fft3d_t(psx_ctx* ctx, cplx<T>* g, int N) {
+  const FftPlan* p;
+  PSX_TRY(get_fft_plan(ctx, N, &p));
+  for (int axis = 0; axis < 3; ++axis)
+    PSX_TRY((run_pass<T, S, false>(ctx, p, g, g, nullptr, nullptr, 0.0, axis)));
+  return PSX_OK;
+}
+
+int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec) {
+  if (prec == PSX_PREC_F64) {
+    return sign > 0 ? fft3d_t<double, 1>(ctx, (cplx<double>*)grid, N)
+                    : fft3d_t<double, -1>(ctx, (cplx<double>*)grid, N);
+  }
+  return sign > 0 ? fft3d_t<float, 1>(ctx, (cplx<float>*)grid, N)
+                  : fft3d_t<float, -1>(ctx, (cplx<float>*)grid, N);
+}
+
+template <typename T>
+static int fft3d_fixed_t(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
+                         cplx<T>* g, int N) {
+  const FftPlan* p;
+  PSX_TRY(get_fft_plan(ctx, N, &p));
+  PSX_TRY((run_pass<T, 1, true>(ctx, p, nullptr, g, ga, gb, inv_scale, 0)));
+  PSX_TRY((run_pass<T, 1, false>(ctx, p, g, g, nullptr, nullptr, 0.0, 1)));
+  PSX_TRY((run_pass<T, 1, false>(ctx, p, g, g, nullptr, nullptr, 0.0, 2)));
+  return PSX_OK;
+}
+
+int launch_fft3d_from_fixed(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
+                            void* grid, int N, int prec) {
+  if (prec == PSX_PREC_F64) return fft3d_fixed_t<double>(ctx, ga, gb, inv_scale, (cplx<double>*)grid, N);
+  return fft3d_fixed_t<float>(ctx, ga, gb, inv_scale, (cplx<float>*)grid, N);
+}
+
+// ------------------------------------------------------------ shell passes --
+// meta per slot: {j, rmax, kxw, unused}
+struct ShellDev {
+  const int* meta;
+  const int64_t* off;
+  const int64_t* mmax;
+  int step;
+  int Nh;
+};
+
+__device__ __forceinline__ int signed_f(int n, int N) { return n <= N / 2 ? n : n - N; }
+
+// z pass: tile (s, ky, kx0): 16 kx columns of one ky row, masked to shell j.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restrict__ H,
+                                                         cplx<T>* __restrict__ T1,
+                                                         const Tile3* __restrict__ tiles,
+                                                         ShellDev sd, FftShape sh,
+                                                         const cplx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int N = sh.N, Nh = sd.Nh, lstride = N + 1;
+  const Tile3 t = tiles[blockIdx.x];
+  const int s = t.s, ky = t.a, kx0 = t.b;
+  const int j = sd.meta[4 * s], kxw = sd.meta[4 * s + 2];
+  const int kys = signed_f(ky, N);
+  constexpr int NL = TileCfg<T>::LINES;  // kx columns per tile (the plan's kt)
+  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
+    const int n = idx / NL, c = idx - n * NL;
+    const int kx = kx0 + c;
+    cplx<T> v = mk(T(0), T(0));
+    if (kx < Nh) {
+      const int kzs = signed_f(n, N);
+      const int64_t m = (int64_t)kx * kx + (int64_t)kys * kys + (int64_t)kzs * kzs;
+      if (shell_of_m(m, sd.step) == j) v = H[((int64_t)n * N + ky) * Nh + kx];
+    }
+    lds[c * lstride + n] = v;
+  }
+  __syncthreads();
+  fft_tile<-1, T, TileCfg<T>::ELEMS>(lds, sh, NL, lstride, tw);
+  cplx<T>* dst = T1 + sd.off[s] + (int64_t)ky * kxw + kx0;
+  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
+    const int n = idx / NL, c = idx - n * NL;
+    dst[(int64_t)n * N * kxw + c] = lds[c * lstride + n];
+  }
+}
+
+// y pass: tile (s, z, kx0): 16 kx columns at one z; rows (ky, kx-tile) that
+// the z pass skipped are zero.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restrict__ T1,
+                                                         cplx<T>* __restrict__ T2,
+                                                         const Tile3* __restrict__ tiles,
+                                                         ShellDev sd, FftShape sh,
+                                                         const cplx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int N = sh.N, lstride = N + 1;
+  const Tile3 t = tiles[blockIdx.x];
+  const int s = t.s, z = t.a, kx0 = t.b;
+  const int kxw = sd.meta[4 * s + 2];
+  const int64_t mm = sd.mmax[s];
+  const int64_t base = sd.off[s] + (int64_t)z * N * kxw + kx0;
+  constexpr int NL = TileCfg<T>::LINES;
+  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
+    const int n = idx / NL, c = idx - n * NL;
+    const int kys = signed_f(n, N);
+    cplx<T> v = mk(T(0), T(0));
+    if ((int64_t)kx0 * kx0 + (int64_t)kys * kys <= mm) v = T1[base + (int64_t)n * kxw + c];
+    lds[c * lstride + n] = v;
+  }
+  __syncthreads();
+  fft_tile<-1, T, TileCfg<T>::ELEMS>(lds, sh, NL, lstride, tw);
+  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
+    const int n = idx / NL, c = idx - n * NL;
+    T2[base + (int64_t)n * kxw + c] = lds[c * lstride + n];
+  }
+}
+
+// x pass (Hermitian -> real): tile (s, z, y0) covers rows y0..y0+15 as 8 pairs;
+// each pair is one complex FFT of X1 + i X2 whose real/imag parts are the two
+// real rows.  kx = 0 and kx = N/2 keep only their real part, which makes the
+// result exactly Re FFT of the Hermitian extension.
+template <typename T, typename TO>
+__global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restrict__ T2,
+                                                         TO* __restrict__ D,
+                                                         const Tile3* __restrict__ tiles,
+                                                         ShellDev sd, FftShape sh,
+                                                         const cplx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int N = sh.N, lstride = N + 1, hN = N / 2;
+  const Tile3 t = tiles[blockIdx.x];
+  const int s = t.s, z = t.a, y0 = t.b;
+  const int kxw = sd.meta[4 * s + 2];
+  const int kxl = min(kxw, hN + 1);
+  const int npair = min(8, (N - y0) / 2);
+  for (int idx = threadIdx.x; idx < npair * N; idx += FFT_THREADS) {
+    const int p = idx / N, n = idx - p * N;
+    lds[p * lstride + n] = mk(T(0), T(0));
+  }
+  __syncthreads();
+  const int64_t base = sd.off[s] + (int64_t)z * N * kxw;
+  for (int idx = threadIdx.x; idx < npair * kxl; idx += FFT_THREADS) {
+    const int p = idx / kxl, kx = idx - p * kxl;
+    const int y1 = y0 + 2 * p;
+    cplx<T> a = T2[base + (int64_t)y1 * kxw + kx];
+    cplx<T> b = T2[base + (int64_t)(y1 + 1) * kxw + kx];
+    cplx<T>* L = lds + p * lstride;
+    if (kx == 0 || kx == hN) {
+      L[kx] = mk(a.re, b.re);
+    } else {
+      L[kx] = mk(a.re - b.im, a.im + b.re);
+      L[N - kx] = mk(a.re + b.im, b.re - a.im);
+    }
+  }
+  __syncthreads();
+  fft_tile<-1, T, TileCfg<T>::ELEMS>(lds, sh, npair, lstride, tw);
+  const int64_t G = (int64_t)N * N * N;
+  TO* dst = D + (int64_t)s * G + ((int64_t)z * N + y0) * N;
+  for (int idx = threadIdx.x; idx < npair * N; idx += FFT_THREADS) {
+    const int p = idx / N, n = idx - p * N;
+    const cplx<T> v = lds[p * lstride + n];
+    dst[(int64_t)(2 * p) * N + n] = (TO)v.re;
+    dst[(int64_t)(2 * p + 1) * N + n] = (TO)v.im;
+  }
+}
+
+template <typename T>
+static int shell_fields_t(psx_ctx* ctx, const cplx<T>* H, int N, int step, int ncut, int nmax,
+                          void* dout) {
+  const FftPlan* p;
+  PSX_TRY(get_fft_plan(ctx, N, &p));
+  const ShellPlan* sp;
+  PSX_TRY(get_shell_plan(ctx, N, step, ncut, nmax, TileCfg<T>::LINES, &sp));
+  void *t1 = nullptr, *t2 = nullptr;
+  PSX_TRY(ctx_buffer(ctx, sizeof(T) == 4 ? "shell_t1_f32" : "shell_t1_f64",
+                     (size_t)sp->t_elems * sizeof(cplx<T>), &t1));
+  PSX_TRY(ctx_buffer(ctx, sizeof(T) == 4 ? "shell_t2_f32" : "shell_t2_f64",
+                     (size_t)sp->t_elems * sizeof(cplx<T>), &t2));
+  ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
+  const size_t lds16 = (size_t)TileCfg<T>::LINES * (N + 1) * sizeof(cplx<T>);
+  const size_t lds8 = (size_t)8 * (N + 1) * sizeof(cplx<T>);
+  const cplx<T>* tw = tw_of<T>(p);
+  if (sp->nz_tiles > 0) {
+    PSX_TRY(set_lds(k_shell_z<T>, lds16));
+    hipLaunchKernelGGL(k_shell_z<T>, dim3(sp->nz_tiles), dim3(FFT_THREADS), lds16, ctx->stream, H,
+                       (cplx<T>*)t1, sp->d_ztiles, sd, p->shape, tw);
+    PSX_CHECK_LAUNCH();
+  }
+  if (sp->ny_tiles > 0) {
+    PSX_TRY(set_lds(k_shell_y<T>, lds16));
+    hipLaunchKernelGGL(k_shell_y<T>, dim3(sp->ny_tiles), dim3(FFT_THREADS), lds16, ctx->stream,
+                       (const cplx<T>*)t1, (cplx<T>*)t2, sp->d_ytiles, sd, p->shape, tw);
+    PSX_CHECK_LAUNCH();
+  }
+  if (sp->nx_tiles > 0) {
+    PSX_TRY(set_lds(k_shell_x<T, T>, lds8));
+    hipLaunchKernelGGL((k_shell_x<T, T>), dim3(sp->nx_tiles), dim3(FFT_THREADS), lds8, ctx->stream,
+                       (const cplx<T>*)t2, (T*)dout, sp->d_xtiles, sd, p->shape, tw);
+    PSX_CHECK_LAUNCH();
+  }
+  return PSX_OK;
+}
+
+int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                        int prec, void* dout) {
+  if (prec == PSX_PREC_F64)
+    return shell_fields_t<double>(ctx, (const cplx<double>*)half, N, step, ncut, nmax, dout);
+  return shell_fields_t<float>(ctx, (const cplx<float>*)half, N, step, ncut, nmax, dout);
+}
+
+}  // namespace psx

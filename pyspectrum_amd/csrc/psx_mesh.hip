@@ -1,0 +1,357 @@
+// Mesh front end of the periodic-box estimator: PCS assignment with
+// interlacing, the interlacing/deconvolution combination, and the half <->
+// full field conversions.
+#include "psx_ctx.h"
+
+namespace psx {
+
+// ------------------------------------------------------------ assignment --
+// Per-axis PCS stencil of estimator.f:302-378, replayed with the reference's
+// float32 operation order (no contraction into FMA): rx = kf_ks*x + 1 is the
+// 1-based grid coordinate; grid A uses rx, grid B uses rx + 0.5.  Returns the
+// 0-based base cell (already wrapped) and the four weights (x6) for cells
+// base-1 .. base+2.
+__device__ __forceinline__ float pcs_coord(float r, float kf_ks, bool shift, int N, int* c) {
+  float rx = __fadd_rn(__fmul_rn(kf_ks, r), 1.0f);
+  if (shift) rx = __fadd_rn(rx, 0.5f);
+  const int i1 = (int)rx;  // rx >= 1: truncation = floor
+  *c = (i1 - 1) % N;
+  return __fsub_rn(rx, (float)i1);  // exact
+}
+
+// float32 weights, the reference's operation order
+__device__ __forceinline__ void pcs_weights(float h, float* wts) {
+  const float h2 = __fmul_rn(h, h);
+  const float t = __fsub_rn(1.0f, h);
+  const float wm2 = __fmul_rn(__fmul_rn(t, t), t);
+  const float wm1 = __fadd_rn(4.0f, __fmul_rn(__fsub_rn(__fmul_rn(3.0f, h), 6.0f), h2));
+  const float wp2 = __fmul_rn(h2, h);
+  const float wp1 = __fsub_rn(__fsub_rn(__fsub_rn(6.0f, wm2), wm1), wp2);
+  wts[0] = wm2; wts[1] = wm1; wts[2] = wp1; wts[3] = wp2;
+}
+
+// float64 weights of the same float32 cell offset (FP64 pipeline)
+__device__ __forceinline__ void pcs_weights(float hf, double* wts) {
+  const double h = (double)hf;
+  const double h2 = h * h;
+  const double t = 1.0 - h;
+  const double wm2 = t * t * t;
+  const double wm1 = 4.0 + (3.0 * h - 6.0) * h2;
+  const double wp2 = h2 * h;
+  wts[0] = wm2; wts[1] = wm1; wts[2] = 6.0 - wm2 - wm1 - wp2; wts[3] = wp2;
+}
+
+template <typename W> __device__ __forceinline__ W wmul(W a, W b);
+template <> __device__ __forceinline__ float wmul<float>(float a, float b) { return __fmul_rn(a, b); }
+template <> __device__ __forceinline__ double wmul<double>(double a, double b) { return a * b; }
+
+// One thread per particle; 2 x 64 contributions are added as int64 fixed
+// point (scale 2^e), so the grid is independent of the order of the adds.
+template <typename W>
+__global__ __launch_bounds__(256) void k_assign(const float* __restrict__ xyz, const float* __restrict__ w,
+                                                int64_t np, int N, float kf_ks, float clip_max,
+                                                double scale, unsigned long long* __restrict__ ga,
+                                                unsigned long long* __restrict__ gb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  float p[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float v = xyz[(int64_t)d * np + i];
+    // np.clip to [0, clip_max]; NaN is sent to 0 so no cell index leaves the grid
+    v = (v >= 0.0f) ? v : 0.0f;
+    v = (v <= clip_max) ? v : clip_max;
+    p[d] = v;
+  }
+  const W we = (W)(w ? w[i] : 1.0f);
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    int cx, cy, cz;
+    W wx[4], wy[4], wz[4];
+    pcs_weights(pcs_coord(p[0], kf_ks, g == 1, N, &cx), wx);
+    pcs_weights(pcs_coord(p[1], kf_ks, g == 1, N, &cy), wy);
+    pcs_weights(pcs_coord(p[2], kf_ks, g == 1, N, &cz), wz);
+    unsigned long long* grid = g == 0 ? ga : gb;
+    int ix[4], iy[4], iz[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      ix[a] = (cx + a - 1 + N) % N;
+      iy[a] = (cy + a - 1 + N) % N;
+      iz[a] = (cz + a - 1 + N) % N;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int64_t row = ((int64_t)iz[c] * N + iy[b]) * N;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          // estimator.f:380-508: hx*hy*hz*we, left to right
+          const W v = wmul(wmul(wmul(wx[a], wy[b]), wz[c]), we);
+          const long long q = __double2ll_rn((double)v * scale);
+          atomicAdd(grid + row + ix[a], (unsigned long long)q);
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// Fixed-point exponent: every cell is bounded by 64 * sum|w| (per-axis PCS
+// weight <= 4), keep that below 2^62.
+int fixed_point_exponent(double wsum_abs) {
+  double bound = 64.0 * (wsum_abs > 1.0 ? wsum_abs : 1.0);
+  int e = 62 - (int)ceil(log2(bound));
+  if (e > 40) e = 40;
+  if (e < 0) e = 0;
+  return e;
+}
+
+int launch_assign(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N, float kf_ks,
+                  float clip_max, double scale, int64_t* ga, int64_t* gb, int prec) {
+  const int64_t G = (int64_t)N * N * N;
+  hipLaunchKernelGGL(k_fill_u64, dim3(2048), dim3(256), 0, ctx->stream, (unsigned long long*)ga, G);
+  PSX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_fill_u64, dim3(2048), dim3(256), 0, ctx->stream, (unsigned long long*)gb, G);
+  PSX_CHECK_LAUNCH();
+  if (np > 0) {
+    if (prec == PSX_PREC_F64)
+      hipLaunchKernelGGL(k_assign<double>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
+                         xyz, w, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
+                         (unsigned long long*)gb);
+    else
+      hipLaunchKernelGGL(k_assign<float>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
+                         xyz, w, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
+                         (unsigned long long*)gb);
+    PSX_CHECK_LAUNCH();
+  }
+  return PSX_OK;
+}
+
+// Fixed-point grids -> interleaved complex grid (re = A, im = B).
+template <typename T>
+__global__ void k_fixed_to_complex(const int64_t* __restrict__ ga, const int64_t* __restrict__ gb,
+                                   double inv_scale, cplx<T>* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = mk((T)((double)ga[i] * inv_scale), (T)((double)gb[i] * inv_scale));
+}
+
+int launch_fixed_to_complex(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
+                            void* grid, int N, int prec) {
+  const int64_t n = (int64_t)N * N * N;
+  const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), 8192);
+  if (prec == PSX_PREC_F64)
+    hipLaunchKernelGGL(k_fixed_to_complex<double>, dim3(blocks), dim3(256), 0, ctx->stream, ga, gb,
+                       inv_scale, (cplx<double>*)grid, n);
+  else
+    hipLaunchKernelGGL(k_fixed_to_complex<float>, dim3(blocks), dim3(256), 0, ctx->stream, ga, gb,
+                       inv_scale, (cplx<float>*)grid, n);
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+// Unit field delta = 1 (the shell indicators of _counts_Bk123), complex double.
+__global__ void k_fill_unit(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    p[2 * i] = 1.0;
+    p[2 * i + 1] = 0.0;
+  }
+}
+
+int launch_fill_unit(psx_ctx* ctx, double* half, int64_t n) {
+  hipLaunchKernelGGL(k_fill_unit, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 8192)), dim3(256),
+                     0, ctx->stream, half, n);
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+// --------------------------------------------------------------- fcomb ----
+// estimator.f:605-675.  One thread per loop iteration (ix, iy, iz) in
+// [0, N/2]^3.  The eight elements {ix,-ix} x {iy,-iy} x {iz,-iz} are read,
+// combined, and the eight stores are replayed in the reference's order on a
+// register copy in which aliased indices (0 and N/2) share a slot, so the
+// value that survives is the reference's.  Only the x <= N/2 elements are
+// written (the half field).  rec[n] is the reference's twiddle recurrence
+// xrec = recx^n (complex double); W is the per-axis PCS window.
+template <typename T> __device__ __forceinline__ T window_val(const float* Wf, const double* Wd, int n);
+template <> __device__ __forceinline__ float window_val<float>(const float* Wf, const double*, int n) { return Wf[n]; }
+template <> __device__ __forceinline__ double window_val<double>(const float*, const double* Wd, int n) { return Wd[n]; }
+
+template <typename T>
+__device__ __forceinline__ cplx<T> slot_get(const cplx<T> (&m)[8], int s) {
+  cplx<T> r = m[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) if (s == k) r = m[k];
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void slot_put(cplx<T> (&m)[8], int s, cplx<T> v) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) if (s == k) m[k] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fcomb(const cplx<T>* __restrict__ F, cplx<T>* __restrict__ Hh, int N,
+                                               T cf, const c64* __restrict__ rec,
+                                               const float* __restrict__ Wf,
+                                               const double* __restrict__ Wd) {
+  const int Nh = N / 2 + 1;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)Nh * Nh * Nh) return;
+  const int ix = (int)(id % Nh);
+  const int iy = (int)((id / Nh) % Nh);
+  const int iz = (int)(id / ((int64_t)Nh * Nh));
+  const int xs[2] = {ix, (N - ix) % N};
+  const int ys[2] = {iy, (N - iy) % N};
+  const int zs[2] = {iz, (N - iz) % N};
+  const bool ax = xs[0] == xs[1], ay = ys[0] == ys[1], az = zs[0] == zs[1];
+  // canonical slot of (xb, yb, zb)
+  auto canon = [&](int xb, int yb, int zb) {
+    return ((xb && !ax) ? 4 : 0) + ((yb && !ay) ? 2 : 0) + ((zb && !az) ? 1 : 0);
+  };
+  cplx<T> v[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int xb = s >> 2, yb = (s >> 1) & 1, zb = s & 1;
+    v[s] = F[((int64_t)zs[zb] * N + ys[yb]) * N + xs[xb]];
+  }
+  // twiddles in double (estimator.f:650-653), rounded to T
+  const c64 xr = rec[ix], yr = rec[iy], zr = rec[iz];
+  const c64 ixr = mk(-xr.im, xr.re);
+  const c64 cma = ixr * yr * zr;
+  const c64 cmb = ixr * yr * conj(zr);
+  const c64 cmc = ixr * conj(yr) * zr;
+  const c64 cmd = ixr * conj(yr * zr);
+  const cplx<T> one = mk(T(1), T(0));
+  auto cv = [](c64 a) { return mk((T)a.re, (T)a.im); };
+  const cplx<T> A = cv(cma), B = cv(cmb), C = cv(cmc), D = cv(cmd);
+  const T cfac = cf / (window_val<T>(Wf, Wd, ix) * window_val<T>(Wf, Wd, iy) * window_val<T>(Wf, Wd, iz));
+  const cplx<T> c000 = v[0] * (one - A) + conj(v[7]) * (one + A);
+  const cplx<T> c001 = v[1] * (one - B) + conj(v[6]) * (one + B);
+  const cplx<T> c010 = v[2] * (one - C) + conj(v[5]) * (one + C);
+  const cplx<T> c011 = v[3] * (one - D) + conj(v[4]) * (one + D);
+  cplx<T> m[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) m[s] = v[s];
+  slot_put(m, canon(0, 0, 0), c000 * cfac);
+  slot_put(m, canon(0, 0, 1), c001 * cfac);
+  slot_put(m, canon(0, 1, 0), c010 * cfac);
+  slot_put(m, canon(0, 1, 1), c011 * cfac);
+  slot_put(m, canon(1, 0, 0), conj(slot_get(m, canon(0, 1, 1))));
+  slot_put(m, canon(1, 0, 1), conj(slot_get(m, canon(0, 1, 0))));
+  slot_put(m, canon(1, 1, 0), conj(slot_get(m, canon(0, 0, 1))));
+  slot_put(m, canon(1, 1, 1), conj(slot_get(m, canon(0, 0, 0))));
+  // half-field elements owned by this iteration: x = ix, y in {iy,-iy}, z in {iz,-iz}
+#pragma unroll
+  for (int yb = 0; yb < 2; ++yb) {
+#pragma unroll
+    for (int zb = 0; zb < 2; ++zb) {
+      if ((yb && ay) || (zb && az)) continue;
+      Hh[((int64_t)zs[zb] * N + ys[yb]) * Nh + ix] = slot_get(m, canon(0, yb, zb));
+    }
+  }
+}
+
+int launch_fcomb(psx_ctx* ctx, const void* F, void* Hh, int N, double sumw, int prec,
+                 const void* rec, const float* Wf, const double* Wd) {
+  const int64_t Nh = N / 2 + 1;
+  const int64_t n = Nh * Nh * Nh;
+  // cf = 1/(6^3 * 4 * N) with N = sum w as float32 (estimator.f:615)
+  const float sumw32 = (float)sumw;
+  if (prec == PSX_PREC_F64) {
+    const double cf = 1.0 / (216.0 * 4.0 * (double)sumw32);
+    hipLaunchKernelGGL(k_fcomb<double>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream,
+                       (const cplx<double>*)F, (cplx<double>*)Hh, N, cf, (const c64*)rec, Wf, Wd);
+  } else {
+    const float cf = 1.0f / (864.0f * sumw32);
+    hipLaunchKernelGGL(k_fcomb<float>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream,
+                       (const cplx<float>*)F, (cplx<float>*)Hh, N, cf, (const c64*)rec, Wf, Wd);
+  }
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+// ------------------------------------------------- half <-> full fields ----
+// Hermitian part of an arbitrary full field (any strides, element type TI),
+// kept on x <= N/2: H(k) = (F(k) + conj F(-k)) / 2.  Re FFT of a field equals
+// the FFT of its Hermitian part, so the shell fields of H equal the
+// reference's Re(fftn(F * mask)) exactly in exact arithmetic.
+template <typename TI, typename T>
+__global__ void k_herm_half(const cplx<TI>* __restrict__ F, int64_t sx, int64_t sy, int64_t sz,
+                            cplx<T>* __restrict__ Hh, int N) {
+  const int Nh = N / 2 + 1;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)Nh * N * N) return;
+  const int x = (int)(id % Nh);
+  const int y = (int)((id / Nh) % N);
+  const int z = (int)(id / ((int64_t)Nh * N));
+  const int mx = (N - x) % N, my = (N - y) % N, mz = (N - z) % N;
+  const cplx<TI> a = F[x * sx + y * sy + z * sz];
+  const cplx<TI> b = F[mx * sx + my * sy + mz * sz];
+  Hh[id] = mk((T)(((double)a.re + (double)b.re) * 0.5), (T)(((double)a.im - (double)b.im) * 0.5));
+}
+
+// reflect_delta (pyspectrum.py:1134-1157) to a C-order [ix][iy][iz] full field.
+template <typename T>
+__global__ void k_reflect(const cplx<T>* __restrict__ Hh, cplx<T>* __restrict__ F, int N) {
+  const int Nh = N / 2 + 1, hg = N / 2;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)N * N * N) return;
+  const int z = (int)(id % N);
+  const int y = (int)((id / N) % N);
+  const int x = (int)(id / ((int64_t)N * N));
+  cplx<T> v;
+  if (x < Nh) {
+    v = Hh[((int64_t)z * N + y) * Nh + x];
+    const bool cx = (x == 0 || x == hg), cy = (y == 0 || y == hg), cz = (z == 0 || z == hg);
+    if (cx && cy && cz && !(x == 0 && y == 0 && z == 0)) v.im = T(0);
+  } else {
+    const int mx = N - x, my = (N - y) % N, mz = (N - z) % N;
+    v = conj(Hh[((int64_t)mz * N + my) * Nh + mx]);
+  }
+  F[id] = v;
+}
+
+int launch_herm_half(psx_ctx* ctx, const void* F, int in_prec, int64_t sx, int64_t sy, int64_t sz,
+                     void* Hh, int N, int prec) {
+  const int64_t n = (int64_t)(N / 2 + 1) * N * N;
+  const dim3 grid((unsigned)cdiv(n, 256));
+  if (in_prec == PSX_PREC_F64) {
+    if (prec == PSX_PREC_F64)
+      hipLaunchKernelGGL((k_herm_half<double, double>), grid, dim3(256), 0, ctx->stream,
+                         (const cplx<double>*)F, sx, sy, sz, (cplx<double>*)Hh, N);
+    else
+      hipLaunchKernelGGL((k_herm_half<double, float>), grid, dim3(256), 0, ctx->stream,
+                         (const cplx<double>*)F, sx, sy, sz, (cplx<float>*)Hh, N);
+  } else {
+    if (prec == PSX_PREC_F64)
+      hipLaunchKernelGGL((k_herm_half<float, double>), grid, dim3(256), 0, ctx->stream,
+                         (const cplx<float>*)F, sx, sy, sz, (cplx<double>*)Hh, N);
+    else
+      hipLaunchKernelGGL((k_herm_half<float, float>), grid, dim3(256), 0, ctx->stream,
+                         (const cplx<float>*)F, sx, sy, sz, (cplx<float>*)Hh, N);
+  }
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+int launch_reflect(psx_ctx* ctx, const void* Hh, void* F, int N, int prec) {
+  const int64_t n = (int64_t)N * N * N;
+  if (prec == PSX_PREC_F64)
+    hipLaunchKernelGGL(k_reflect<double>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream,
+                       (const cplx<double>*)Hh, (cplx<double>*)F, N);
+  else
+    hipLaunchKernelGGL(k_reflect<float>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream,
+                       (const cplx<float>*)Hh, (cplx<float>*)F, N);
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+}  // namespace psx

@@ -1,0 +1,285 @@
+"""GPU parity tests: every stage of libpsx.so through the C ABI against the
+oracle (FP64 restatement of the reference) on seeded inputs, and N_tri against
+the reference's own FP64 count caches.
+
+Tolerances (max |gpu - oracle| / max |oracle| unless stated):
+  FP64 pipeline   <= 1e-10   (assignment in fixed point, FP64 FFT/MFMA)
+  FP32 pipeline   <= 1e-5    (north-star FP32 bar)
+  split-BF16      <= 1e-3    (north-star split-BF16 bar)
+  bins, counts, N_tri: bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectrum as S
+from pyspectrum_amd import _lib as L
+from pyspectrum_amd import pyspectrum as P
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-10, "fp32": 1e-5, "bf16x3": 1e-3}
+DEV = torch.device("cuda", 0)
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _h():
+    return L.context(0).handle
+
+
+def _cdt(prec):
+    return torch.complex128 if prec == "fp64" else torch.complex64
+
+
+def _rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _particles(n, Lbox, seed, weights=False):
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(0, Lbox, size=(3, n))
+    w = rng.uniform(0.5, 1.5, size=n) if weights else None
+    return xyz, w
+
+
+# ------------------------------------------------------------------ FFT ----
+@pytest.mark.parametrize("N", [16, 24, 30, 96, 128])
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_fft3d_matches_dft(N, prec, sign):
+    _say("fft3d", N, prec, sign)
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))  # [z][y][x]
+    g = torch.from_numpy(x.astype(np.complex128 if prec == "fp64" else np.complex64)).to(DEV)
+    L.call("psx_fft3d", _h(), L.ptr(g), N, sign, L.PREC[prec])
+    ref = np.fft.ifftn(x) * N ** 3 if sign > 0 else np.fft.fftn(x)
+    assert _rel(g.cpu().numpy(), ref) < (1e-12 if prec == "fp64" else 2e-6)
+
+
+# ----------------------------------------------------------- assignment ----
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_assign_quad_matches_oracle(prec):
+    _say("assign", prec)
+    N, Lbox = 24, 300.0
+    xyz, w = _particles(3000, Lbox, 1, weights=True)
+    xyz[:, :5] = [[-3.0, 300.0, 1e9, 0.0, 299.99999]] * 3   # clipping edge cases
+    xyz32 = S.clip_positions(xyz, Lbox)
+    kf_ks = np.float32(N / Lbox)
+    A, B = S.assign_quad(xyz32, w.astype(np.float32).astype(np.float64), N, kf_ks)
+    x = torch.from_numpy(np.ascontiguousarray(xyz.astype(np.float32))).to(DEV)
+    wd = torch.from_numpy(w.astype(np.float32)).to(DEV)
+    g = torch.empty((N, N, N), dtype=_cdt(prec), device=DEV)
+    L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N, float(kf_ks),
+           float(np.float32(Lbox * (1 - 1e-6))), float(np.abs(w).sum()), L.ptr(g), L.PREC[prec])
+    out = g.cpu().numpy().transpose(2, 1, 0)  # -> [ix, iy, iz]
+    tol = 1e-11 if prec == "fp64" else 2e-7
+    assert _rel(out.real, A) < tol and _rel(out.imag, B) < tol
+
+
+def test_assign_is_deterministic():
+    _say("assign determinism")
+    N, Lbox = 64, 1000.0
+    xyz, _ = _particles(200000, Lbox, 2)
+    x = torch.from_numpy(np.ascontiguousarray(xyz.astype(np.float32))).to(DEV)
+    outs = []
+    for _ in range(2):
+        g = torch.empty((N, N, N), dtype=torch.complex64, device=DEV)
+        L.call("psx_assign_quad", _h(), L.ptr(x), None, x.shape[1], N, float(np.float32(N / Lbox)),
+               float(np.float32(Lbox * (1 - 1e-6))), float(x.shape[1]), L.ptr(g), L.PREC["fp32"])
+        outs.append(g.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+# --------------------------------------------------------------- fcomb -----
+@pytest.mark.parametrize("N", [16, 24, 96])
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_fcomb_matches_oracle(N, prec):
+    _say("fcomb", N, prec)
+    rng = np.random.default_rng(100 + N)
+    F = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))  # [x][y][z]
+    ref = S.fcomb_periodic(F, 321.0, N)[: N // 2 + 1]
+    g = torch.from_numpy(np.ascontiguousarray(F.transpose(2, 1, 0))).to(DEV).to(_cdt(prec))
+    half = torch.empty((N, N, N // 2 + 1), dtype=_cdt(prec), device=DEV)
+    L.call("psx_fcomb_periodic", _h(), L.ptr(g), L.ptr(half), N, 321.0, L.PREC[prec])
+    out = half.cpu().numpy().transpose(2, 1, 0)
+    assert _rel(out, ref) < (1e-12 if prec == "fp64" else 2e-6)
+
+
+# ------------------------------------------------------- FFT_periodic ------
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("N,weights", [(32, False), (48, True)])
+def test_fft_periodic_matches_oracle(prec, N, weights):
+    _say("FFT_periodic", prec, N, weights)
+    Lbox = 500.0
+    xyz, w = _particles(5000, Lbox, 3, weights)
+    ref = S.FFT_periodic(xyz, w, Lbox, N)
+    out = P.FFT_periodic(xyz, w=w, Lbox=Lbox, Ngrid=N, precision=prec)
+    assert out.shape == (N // 2 + 1, N, N)
+    assert _rel(out, ref) < (1e-10 if prec == "fp64" else 1e-5)
+
+
+def test_reflect_matches_oracle():
+    _say("reflect")
+    N = 24
+    rng = np.random.default_rng(5)
+    half = (rng.standard_normal((N // 2 + 1, N, N)) + 1j * rng.standard_normal((N // 2 + 1, N, N))).astype(np.complex64)
+    assert np.array_equal(P.reflect_delta(half, Ngrid=N), S.reflect_delta(half, N))
+
+
+# ------------------------------------------------------- shells / B raw ----
+def _half_of(full, prec):
+    N = full.shape[0]
+    f = torch.from_numpy(full.astype(np.complex128)).to(DEV)
+    half = torch.empty((N, N, N // 2 + 1), dtype=_cdt(prec), device=DEV)
+    sx, sy, sz = f.stride()
+    L.call("psx_hermitian_half", _h(), L.ptr(f), L.PREC["fp64"], sx, sy, sz, L.ptr(half), N, L.PREC[prec])
+    return half
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("N,step,ncut,nmax", [(32, 2, 2, 6), (24, 1, 1, 12), (48, 3, 3, 8)])
+def test_shell_fields_match_oracle(prec, N, step, ncut, nmax):
+    _say("shells", prec, N, step, ncut, nmax)
+    rng = np.random.default_rng(7)
+    full = np.fft.fftn(rng.standard_normal((N, N, N)))
+    D, _ = S.shell_fields(full, step, ncut, nmax)
+    half = _half_of(full, prec)
+    j0 = ncut // step
+    ns = nmax - j0 + 1
+    d = torch.empty((ns, N, N, N), dtype=torch.float64 if prec == "fp64" else torch.float32, device=DEV)
+    L.call("psx_bk_shell_fields", _h(), L.ptr(half), N, step, ncut, nmax, L.ptr(d), L.PREC[prec])
+    got = d.cpu().numpy().transpose(0, 3, 2, 1)  # [s][x][y][z]
+    for s in range(ns):
+        j = j0 + s
+        scale = max(np.max(np.abs(D[j])), 1e-300)
+        assert np.max(np.abs(got[s] - D[j])) / scale < (1e-12 if prec == "fp64" else 3e-6), j
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("N,step,ncut,nmax", [(32, 2, 2, 6), (48, 1, 1, 20), (40, 1, 1, 17)])
+def test_bk_raw_matches_oracle(prec, N, step, ncut, nmax):
+    _say("bk_raw", prec, N, step, ncut, nmax)
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal((N, N, N))
+    full = np.fft.fftn(r + 0.5 * r ** 2)
+    D, p0k = S.shell_fields(full, step, ncut, nmax)
+    I, J, Lx, _ = S.triangles(nmax, ncut, step)
+    ref = np.array([np.einsum("i,i,i", D[i].ravel(), D[j].ravel(), D[l].ravel()) for i, j, l in zip(I, J, Lx)])
+    half = _half_of(full, prec)
+    braw = torch.empty(len(I), dtype=torch.float64, device=DEV)
+    p0 = torch.empty(nmax, dtype=torch.float64, device=DEV)
+    L.call("psx_bk_raw", _h(), L.ptr(half), N, step, ncut, nmax, L.ptr(braw), L.ptr(p0), L.PREC[prec])
+    got = braw.cpu().numpy()
+    # error relative to the typical triangle magnitude (the sums change sign)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) / scale < TOL[prec]
+    j0 = ncut // step
+    assert _rel(p0.cpu().numpy()[j0 - 1 if j0 > 0 else 0:], p0k[j0 - 1 if j0 > 0 else 0:]) < 1e-10
+
+
+def test_bk_raw_is_deterministic():
+    _say("bk_raw determinism")
+    N = 48
+    rng = np.random.default_rng(12)
+    full = np.fft.fftn(rng.standard_normal((N, N, N)))
+    half = _half_of(full, "fp32")
+    outs = []
+    for _ in range(2):
+        braw = torch.empty(150, dtype=torch.float64, device=DEV)
+        L.call("psx_bk_raw", _h(), L.ptr(half), N, 3, 3, 10, L.ptr(braw), None, L.PREC["fp32"])
+        outs.append(braw.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+# --------------------------------------------------------------- N_tri -----
+def _fixture(golden_dir, nmax):
+    g = np.load(os.path.join(golden_dir, "ntri_Ngrid360_Ncut3_step3_Nmax%d.npz" % nmax))
+    return {(a, b, c): (n, v) for a, b, c, n, v in zip(g["i"], g["j"], g["l"], g["ntri"], g["counts"])}
+
+
+@pytest.mark.parametrize("N,nmax", [(96, 10), (360, 10), (360, 40)])
+def test_ntri_bit_exact_against_reference_cache(golden_dir, N, nmax):
+    _say("ntri", N, nmax)
+    ijl = L.triangles(3, 3, nmax)
+    out = torch.empty(len(ijl), dtype=torch.float64, device=DEV)
+    L.call("psx_bk_counts", _h(), N, 3, 3, nmax, L.ptr(out))
+    c = out.cpu().numpy()
+    G = float(N) ** 3
+    ntri = np.rint(c / G).astype(np.int64)
+    assert np.max(np.abs(c / G - ntri)) < 1e-3
+    fx = _fixture(golden_dir, nmax)
+    want = np.array([fx[tuple(t)][0] for t in ijl])
+    assert np.array_equal(ntri, want)
+    if N == 360:
+        # the raw FP64 sums agree with the reference's to FFT round-off
+        raw = np.array([fx[tuple(t)][1] for t in ijl])
+        assert np.max(np.abs(c - raw)) / G < 2e-4
+
+
+# ------------------------------------------------------------- P(k) ------
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_pk_periodic_matches_oracle(prec):
+    _say("Pk_periodic", prec)
+    N, Lbox = 48, 800.0
+    xyz, w = _particles(20000, Lbox, 21, weights=True)
+    ref = S.Pk_periodic(xyz, w, Lbox, N)
+    got = P.Pk_periodic(xyz, w=w, Lbox=Lbox, Ngrid=N, precision=prec)
+    assert np.array_equal(got["counts"], ref["counts"])
+    assert _rel(got["k"], ref["k"]) < 1e-13
+    assert _rel(got["p0k"], ref["p0k"]) < (1e-10 if prec == "fp64" else 1e-5)
+    assert got["p0k_sn"] == pytest.approx(ref["p0k_sn"], rel=1e-15)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("rsd", [0, 1, 2])
+def test_pk_periodic_rsd_matches_oracle(prec, rsd):
+    _say("Pk_periodic_rsd", prec, rsd)
+    N, Lbox = 40, 700
+    xyz, _ = _particles(15000, Lbox, 22)
+    ref = S.Pk_periodic_rsd(xyz, None, Lbox, N, rsd, 10)
+    got = P.Pk_periodic_rsd(xyz, Lbox=Lbox, Ngrid=N, rsd=rsd, Nmubin=10, precision=prec)
+    assert np.array_equal(got["counts"], ref["counts"])
+    assert np.array_equal(got["counts_kmu"], ref["counts_kmu"])
+    tol = 1e-9 if prec == "fp64" else 1e-5
+    for k in ("k", "p0k", "k_kmu", "mu_kmu", "p_kmu"):
+        assert _rel(got[k], ref[k]) < tol, k
+    for k in ("p2k", "p4k"):
+        # Legendre sums cancel; scale by the monopole
+        assert np.max(np.abs(got[k] - ref[k])) <= tol * np.max(np.abs(ref["p0k"] + 1 / ref["meta"]["nbar"])), k
+
+
+# ------------------------------------------------------------- B(k) ------
+def _bk_compare(got, ref, tol):
+    for k in ("i_k1", "i_k2", "i_k3"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.allclose(got["counts"], ref["counts"], rtol=1e-9, atol=0)
+    for k in ("p0k1", "p0k2", "p0k3"):
+        assert _rel(got[k], ref[k]) < tol, k
+    # B is shot-noise subtracted and may cross zero: scale by max(|B|, B_sn)
+    den = np.maximum(np.abs(ref["b123"]), np.abs(ref["b123_sn"]))
+    assert np.max(np.abs(got["b123"] - ref["b123"]) / den) < tol
+    assert _rel(got["b123_sn"], ref["b123_sn"]) < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_bk_periodic_matches_oracle_small(prec):
+    _say("Bk_periodic small", prec)
+    N, Lbox = 48, 600.0
+    rng = np.random.default_rng(31)
+    # clustered: half the points in small clumps
+    c = rng.uniform(0, Lbox, size=(3, 200))
+    pts = [c[:, rng.integers(0, 200, 10000)] + rng.normal(0, 8.0, size=(3, 10000)),
+           rng.uniform(0, Lbox, size=(3, 10000))]
+    xyz = np.concatenate(pts, axis=1) % Lbox
+    counts = S.counts_Bk123(N, 10, 3, 3)
+    # a count that rounds to N_tri = 0 is an empty triangle (both sides)
+    counts[np.rint(counts / N ** 3) == 0] = 0.0
+    ref = S.Bk_periodic(xyz, None, Lbox, N, 3, 3, 10, counts=counts)
+    got = P.Bk_periodic(xyz, Lbox=Lbox, Ngrid=N, step=3, Ncut=3, Nmax=10, precision=prec)
+    _bk_compare(got, ref, TOL[prec])
