@@ -1,0 +1,234 @@
+"""Throughput / latency benchmark of the periodic-box B(k) pipeline on one MI355X.
+
+Workload (one "step" = one box): 1,000,000 lognormal points in a 2500 Mpc/h
+box (Gaussian field with P(k) ~ k^-1.5 exp(-(k/0.5)^2) on a 512^3 mesh, seed 2),
+B(k) at Ngrid=360, step=3, Ncut=3, Nmax=40 (40 shells, 6350 triangles):
+PCS assignment with interlacing -> 3D FFT -> fcomb -> 40 shell fields ->
+triangle contraction -> N_tri normalisation, units and shot noise.  Inputs are
+resident in HBM before timing; N_tri comes from the FP64 count cache (as in
+the reference), computed once before the timed region.
+
+Prints one JSON line: boxes/s of the mixed-precision (split-BF16) path, its
+p50/p99 latency, the FP64 path's boxes/s and the ratio, per-stage times, the
+roofline of the dominant kernel and a CPU baseline (the oracle, timed on a
+bounded sample of the same workload on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"hbm_gbs": 8000.0, "bf16_tflops": 2500.0, "fp32_tflops": 157.3, "fp64_tflops": 78.6}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32", "fp64"])
+    ap.add_argument("--fp64-steps", type=int, default=5)
+    ap.add_argument("--npart", type=int, default=1_000_000)
+    ap.add_argument("--ngrid", type=int, default=360)
+    ap.add_argument("--nmax", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-triangles", type=int, default=12)
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+class Box:
+    """Device-resident inputs and outputs of one B(k) configuration."""
+
+    def __init__(self, xyz, Lbox, Ngrid, step, Ncut, Nmax):
+        import torch
+        from pyspectrum_amd import _lib as L
+        from pyspectrum_amd import pyspectrum as P
+        self.L, self.torch = L, torch
+        self.xyz, self.Lbox, self.N = xyz, Lbox, Ngrid
+        self.step, self.Ncut, self.Nmax = step, Ncut, Nmax
+        self.np_ = xyz.shape[1]
+        self.ijl = L.triangles(step, Ncut, Nmax)
+        self.ntri = len(self.ijl)
+        counts = P._counts_Bk123(Ngrid=Ngrid, Nmax=Nmax, Ncut=Ncut, step=step)
+        tc = P._tri_counts(counts, self.ijl, Ngrid)
+        dev = xyz.device
+        self.cnt = torch.from_numpy(tc).to(dev)
+        self.ijl_d = torch.from_numpy(np.ascontiguousarray(self.ijl)).to(dev)
+        self.out7 = torch.empty((7, self.ntri), dtype=torch.float64, device=dev)
+        self.p0k = torch.empty(Nmax, dtype=torch.float64, device=dev)
+        self.nbar = self.np_ / Lbox ** 3
+        self.kf = 2 * np.pi / Lbox
+        self.ctx = L.context(dev.index or 0)
+
+    def run(self, prec):
+        L = self.L
+        L.call("psx_bk_periodic", self.ctx.handle, L.ptr(self.xyz), None, self.np_, self.N,
+               float(self.Lbox), float(self.np_), float(self.np_), self.step, self.Ncut, self.Nmax,
+               L.ptr(self.cnt), L.ptr(self.ijl_d), float(self.nbar), (2 * np.pi) ** 3, self.kf ** 3,
+               (2 * np.pi) ** 6, self.kf ** 6, L.ptr(self.out7), L.ptr(self.p0k), L.PREC[prec])
+
+
+def time_steps(box, prec, steps, warmup):
+    torch = box.torch
+    for _ in range(warmup):
+        box.run(prec)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        box.run(prec)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    return wall, lat
+
+
+def stage_times(box, prec, reps=3):
+    """Per-stage device times (ms, median of reps) through the staged C ABI."""
+    torch, L = box.torch, box.L
+    N, dev = box.N, box.xyz.device
+    fp = L.PREC[prec]
+    cd = torch.complex128 if prec == "fp64" else torch.complex64
+    rd = torch.float64 if prec == "fp64" else torch.float32
+    j0 = box.Ncut // box.step
+    ns = box.Nmax - j0 + 1
+    half = torch.empty((N, N, N // 2 + 1), dtype=cd, device=dev)
+    D = torch.empty((ns, N, N, N), dtype=rd, device=dev)
+    braw = torch.empty(box.ntri, dtype=torch.float64, device=dev)
+    h = box.ctx.handle
+    stages = {
+        "fft_periodic": lambda: L.call("psx_fft_periodic", h, L.ptr(box.xyz), None, box.np_, N,
+                                       float(box.Lbox), float(box.np_), float(box.np_), L.ptr(half), fp),
+        "shell_fields": lambda: L.call("psx_bk_shell_fields", h, L.ptr(half), N, box.step, box.Ncut,
+                                       box.Nmax, L.ptr(D), fp),
+        "contraction": lambda: L.call("psx_bk_contract", h, L.ptr(D), N, box.step, box.Ncut,
+                                      box.Nmax, L.ptr(braw), fp),
+    }
+    out = {}
+    for name, fn in stages.items():
+        fn()
+        ts = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(); fn(); b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        out[name] = float(np.median(ts))
+    del D
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(xyz_np, Lbox, Ngrid, step, Ncut, Nmax, n_tri_sample):
+    """The oracle (FP64 numpy restatement of the reference path, 1 thread) timed
+    on a bounded sample of the same box: the full front end (assignment, FFT,
+    fcomb, reflect), one shell field and n_tri_sample triangle sums; the box
+    time is extrapolated to all shells and triangles."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import spectrum as S
+    t0 = time.perf_counter()
+    half = S.FFT_periodic(xyz_np, None, Lbox, Ngrid)
+    full = S.reflect_delta(half, Ngrid)
+    irk = S.shell_index(Ngrid, step)
+    t_front = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    d = [np.fft.fftn(np.where(irk == j, full, 0)).real.ravel() for j in (Nmax,)]
+    t_shell = time.perf_counter() - t0
+    d3 = [d[0], d[0] * 0.5, d[0] * 0.25]
+    t0 = time.perf_counter()
+    for _ in range(n_tri_sample):
+        np.einsum("i,i,i", d3[0], d3[1], d3[2])
+    t_tri = (time.perf_counter() - t0) / n_tri_sample
+    ns = Nmax - Ncut // step + 1
+    ntri = len(S.triangles(Nmax, Ncut, step)[0])
+    box_s = t_front + ns * t_shell + ntri * t_tri
+    sample_s = t_front + t_shell + n_tri_sample * t_tri
+    return {"value": 1.0 / box_s, "unit": "boxes/s", "cores": 1, "kind": "port",
+            "sample": ("oracle (FP64 numpy restatement) on the same 1e6-point box: full front end "
+                       "%.1fs, 1 of %d shell FFTs %.1fs, %d of %d triangle einsums %.3fs each "
+                       "(%.0fs sampled); box time extrapolated to %.0fs"
+                       % (t_front, ns, t_shell, n_tri_sample, ntri, t_tri, sample_s, box_s))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box runs on a single MI355X; "
+              "--gpus %d is not supported (multi-GPU sharding is design-only)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from pyspectrum_amd import _lib as L
+    from pyspectrum_amd import mocks
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    Lbox, N, step, Ncut, Nmax = 2500.0, args.ngrid, 3, 3, args.nmax
+    t0 = time.perf_counter()
+    xyz, A = mocks.lognormal_box(args.npart, Lbox, seed=2)
+    torch.cuda.synchronize()
+    if args.verbose:
+        print("mock generated in %.1fs (A=%.4g)" % (time.perf_counter() - t0, A), file=sys.stderr, flush=True)
+    box = Box(xyz, Lbox, N, step, Ncut, Nmax)
+    if args.verbose:
+        print("counts ready", file=sys.stderr, flush=True)
+
+    wall, lat = time_steps(box, args.precision, args.steps, args.warmup)
+    value = args.steps / wall
+    wall64, lat64 = time_steps(box, "fp64", args.fp64_steps, 1)
+    v64 = args.fp64_steps / wall64
+    st = stage_times(box, args.precision)
+    st64 = stage_times(box, "fp64")
+    if args.verbose:
+        print("stages", st, st64, file=sys.stderr, flush=True)
+
+    # roofline of the dominant kernel (the triangle contraction)
+    G = float(N) ** 3
+    ijl = box.ijl
+    npairs = len({(int(i), int(j)) for i, j, _ in ijl})
+    flops = G * (npairs + 2 * box.ntri)
+    t_c = st["contraction"] / 1e3
+    achieved = flops / t_c / 1e12
+    peak = PEAK["bf16_tflops"] if args.precision == "bf16x3" else (
+        PEAK["fp32_tflops"] if args.precision == "fp32" else PEAK["fp64_tflops"])
+    roofline = {"bound": "mfma", "kernel": "k_contract", "achieved": round(achieved, 2),
+                "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        xyz_np = xyz.double().cpu().numpy()
+        cpu = cpu_baseline(xyz_np, Lbox, N, step, Ncut, Nmax, args.cpu_sample_triangles)
+    out = {
+        "metric": "B(k) boxes/s at Ngrid=360 + p50/p99 latency, mixed-precision vs FP64 GPU",
+        "value": round(value, 3), "unit": "boxes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16x3+f32", "fp32": "f32", "fp64": "f64"}[args.precision],
+        "data": "synthetic lognormal mock (seed 2), generated on the GPU",
+        "config": {"workload": "single clustered box: 1e6 lognormal points, Lbox=2500, Ngrid=%d, "
+                               "step=3, Ncut=3, Nmax=%d (%d triangles)" % (N, Nmax, box.ntri),
+                   "precision": args.precision},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "fp64_boxes_per_s": round(v64, 3),
+        "fp64_p50_ms": round(float(np.percentile(lat64, 50)), 3),
+        "mixed_vs_fp64": round(value / v64, 3),
+        "stages_ms": {k: round(v, 3) for k, v in st.items()},
+        "stages_fp64_ms": {k: round(v, 3) for k, v in st64.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

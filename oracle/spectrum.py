@@ -144,13 +144,15 @@ def assign_quad(xyz32, w, Ngrid, kf_ks32):
         ix, wx = _pcs_axis(xyz32[0], kf_ks32, Ngrid, shift)
         iy, wy = _pcs_axis(xyz32[1], kf_ks32, Ngrid, shift)
         iz, wz = _pcs_axis(xyz32[2], kf_ks32, Ngrid, shift)
-        flat = np.zeros(G, dtype=np.float64)
+        idx, val = [], []
         for a in range(4):
             for b in range(4):
                 wab = wx[a] * wy[b]
                 iab = (ix[a] * Ngrid + iy[b]) * Ngrid
                 for c in range(4):
-                    flat += np.bincount(iab + iz[c], weights=wab * wz[c] * w, minlength=G)
+                    idx.append(iab + iz[c])
+                    val.append(wab * wz[c] * w)
+        flat = np.bincount(np.concatenate(idx), weights=np.concatenate(val), minlength=G)
         grids.append(flat.reshape(Ngrid, Ngrid, Ngrid))
     return grids[0], grids[1]
 

@@ -292,7 +292,12 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
     const int w = pt_wave[tpt[t]], q = pt_q[tpt[t]];
     const int slot = wave_slot[(w * maxpt + q) * 4 + tlt[t]];
     const int row = trow[t], col = tcol[t];
-    const int lane = col + 16 * (row / 4), reg = row % 4;
+    // 16x16 MFMA output layout (measured on gfx950): f32 outputs (f32 and bf16
+    // inputs) hold row r in lane col + 16*(r/4), register r%4; the f64 MFMA
+    // holds it in lane col + 16*(r%4), register r/4.
+    const bool f64 = prec == PSX_PREC_F64;
+    const int lane = col + 16 * (f64 ? row % 4 : row / 4);
+    const int reg = f64 ? row / 4 : row % 4;
     tri_loc[t] = slot * 256 + lane * 4 + reg;
     tri_grp[t] = w / WPG;
   }

@@ -120,19 +120,24 @@ __global__ __launch_bounds__(256) void k_pk_rsd(const cplx<T>* __restrict__ Hh, 
     const int ix = (int)(id % N);
     const int iy = (int)((id / N) % N);
     const int iz = (int)(id / ((int64_t)N * N));
-    const float rkx = (float)fort_freq(ix, N), rky = (float)fort_freq(iy, N), rkz = (float)fort_freq(iz, N);
-    const float rk = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(rkx, rkx), __fmul_rn(rky, rky)), __fmul_rn(rkz, rkz)));
-    const int imk = (int)roundf(__fdiv_rn(__fmul_rn(fnb, rk), fh));
-    if (!(imk <= nb && imk != 0)) continue;
-    const float cot1 = __fdiv_rn(rkz, rk);
-    const float sit1 = __fsqrt_rn(__fsub_rn(1.0f, __fmul_rn(cot1, cot1)));
-    float cc = 0.0f;
-    if (sit1 > 0.0f) {
-      const float den = __fmul_rn(rk, sit1);
-      const float cp = __fdiv_rn(rkx, den), sp = __fdiv_rn(rky, den);
-      cc = __fadd_rn(__fmul_rn(c.sph, sp), __fmul_rn(c.cph, cp));
+    float rk, mu32;
+    int imk;
+    {
+#pragma clang fp contract(off)
+      const float rkx = (float)fort_freq(ix, N), rky = (float)fort_freq(iy, N), rkz = (float)fort_freq(iz, N);
+      rk = cr_sqrtf(rkx * rkx + rky * rky + rkz * rkz);
+      imk = (int)roundf(cr_divf(fnb * rk, fh));  // NINT: half away from zero
+      if (!(imk <= nb && imk != 0)) continue;
+      const float cot1 = cr_divf(rkz, rk);
+      const float sit1 = cr_sqrtf(1.0f - cot1 * cot1);
+      float cc = 0.0f;
+      if (sit1 > 0.0f) {
+        const float den = rk * sit1;
+        const float cp = cr_divf(rkx, den), sp = cr_divf(rky, den);
+        cc = c.sph * sp + c.cph * cp;
+      }
+      mu32 = c.cth * cot1 + (c.sth * sit1) * cc;
     }
-    const float mu32 = __fadd_rn(__fmul_rn(c.cth, cot1), __fmul_rn(__fmul_rn(c.sth, sit1), cc));
     const double mu = (double)mu32;
     const int imu = (int)((fabs(mu) + md) / md);
     const double mu2 = mu * mu;
@@ -141,9 +146,14 @@ __global__ __launch_bounds__(256) void k_pk_rsd(const cplx<T>* __restrict__ Hh, 
     cplx<T> ct;
     if (ix <= N / 2) ct = Hh[((int64_t)iz * N + iy) * Nh + ix];
     else ct = Hh[((int64_t)((N - iz) % N) * N + (N - iy) % N) * Nh + (N - ix)];
-    const float a = hypotf((float)ct.re, (float)ct.im);
-    const double pk = (double)__fmul_rn(a, a);
-    const double kk = (double)__fmul_rn(c.kf, rk);
+    double pk;
+    if constexpr (sizeof(T) == 8) {
+      pk = (double)ct.re * (double)ct.re + (double)ct.im * (double)ct.im;
+    } else {  // the reference's float32 cabs(ct)**2
+      const float a = hypotf(ct.re, ct.im);
+      pk = (double)(a * a);
+    }
+    const double kk = (double)(c.kf * rk);
     const int b = imk - 1;
     atomicAdd(&acc[0 * nb + b], kk);
     atomicAdd(&acc[1 * nb + b], pk);

@@ -72,6 +72,29 @@ __host__ __device__ __forceinline__ int64_t isqrt64(int64_t x) {
   return r;
 }
 
+// Correctly rounded float32 division and square root, independent of how the
+// compiler lowers '/' and sqrtf.  The quotient goes through IEEE float64
+// division; rounding a 53-bit correctly rounded result to 24 bits equals
+// direct rounding because 53 >= 2*24 + 2 (double rounding is innocuous).
+// The square root starts from the float64 root and is then corrected with
+// exact midpoint tests (midpoints have 25 bits, so their squares are exact in
+// float64).
+__device__ __forceinline__ float cr_divf(float a, float b) {
+  return (float)((double)a / (double)b);
+}
+
+__device__ __forceinline__ float cr_sqrtf(float x) {
+  if (!(x > 0.0f)) return sqrtf(x);  // 0, negative, NaN
+  float r = (float)sqrt((double)x);
+  const double xd = (double)x;
+  const float up = nextafterf(r, INFINITY), dn = nextafterf(r, 0.0f);
+  const double mhi = 0.5 * ((double)r + (double)up);
+  const double mlo = 0.5 * ((double)r + (double)dn);
+  if (mhi * mhi < xd || (mhi * mhi == xd && (__float_as_uint(up) & 1u) == 0)) r = up;
+  else if (mlo * mlo > xd || (mlo * mlo == xd && (__float_as_uint(dn) & 1u) == 0)) r = dn;
+  return r;
+}
+
 // Shell index of |k|^2 = m: int(sqrt(m)/step + 0.5) in exact integers
 // (largest n with (2n-1)*step <= 2*sqrt(m)); step = 1 gives the P(k) bins.
 __host__ __device__ __forceinline__ int shell_of_m(int64_t m, int step) {

@@ -11,22 +11,26 @@ namespace psx {
 // 1-based grid coordinate; grid A uses rx, grid B uses rx + 0.5.  Returns the
 // 0-based base cell (already wrapped) and the four weights (x6) for cells
 // base-1 .. base+2.
+// (HIP's __fmul_rn/__fadd_rn are plain operators, so FMA contraction is
+// switched off explicitly: the reference multiplies and adds separately.)
 __device__ __forceinline__ float pcs_coord(float r, float kf_ks, bool shift, int N, int* c) {
-  float rx = __fadd_rn(__fmul_rn(kf_ks, r), 1.0f);
-  if (shift) rx = __fadd_rn(rx, 0.5f);
+#pragma clang fp contract(off)
+  float rx = kf_ks * r + 1.0f;
+  if (shift) rx = rx + 0.5f;
   const int i1 = (int)rx;  // rx >= 1: truncation = floor
   *c = (i1 - 1) % N;
-  return __fsub_rn(rx, (float)i1);  // exact
+  return rx - (float)i1;  // exact
 }
 
 // float32 weights, the reference's operation order
 __device__ __forceinline__ void pcs_weights(float h, float* wts) {
-  const float h2 = __fmul_rn(h, h);
-  const float t = __fsub_rn(1.0f, h);
-  const float wm2 = __fmul_rn(__fmul_rn(t, t), t);
-  const float wm1 = __fadd_rn(4.0f, __fmul_rn(__fsub_rn(__fmul_rn(3.0f, h), 6.0f), h2));
-  const float wp2 = __fmul_rn(h2, h);
-  const float wp1 = __fsub_rn(__fsub_rn(__fsub_rn(6.0f, wm2), wm1), wp2);
+#pragma clang fp contract(off)
+  const float h2 = h * h;
+  const float t = 1.0f - h;
+  const float wm2 = (t * t) * t;
+  const float wm1 = 4.0f + (3.0f * h - 6.0f) * h2;
+  const float wp2 = h2 * h;
+  const float wp1 = ((6.0f - wm2) - wm1) - wp2;
   wts[0] = wm2; wts[1] = wm1; wts[2] = wp1; wts[3] = wp2;
 }
 
@@ -42,7 +46,7 @@ __device__ __forceinline__ void pcs_weights(float hf, double* wts) {
 }
 
 template <typename W> __device__ __forceinline__ W wmul(W a, W b);
-template <> __device__ __forceinline__ float wmul<float>(float a, float b) { return __fmul_rn(a, b); }
+template <> __device__ __forceinline__ float wmul<float>(float a, float b) { return a * b; }
 template <> __device__ __forceinline__ double wmul<double>(double a, double b) { return a * b; }
 
 // One thread per particle; 2 x 64 contributions are added as int64 fixed

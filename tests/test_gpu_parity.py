@@ -78,7 +78,7 @@ def test_assign_quad_matches_oracle(prec):
     L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N, float(kf_ks),
            float(np.float32(Lbox * (1 - 1e-6))), float(np.abs(w).sum()), L.ptr(g), L.PREC[prec])
     out = g.cpu().numpy().transpose(2, 1, 0)  # -> [ix, iy, iz]
-    tol = 1e-11 if prec == "fp64" else 2e-7
+    tol = 1e-11 if prec == "fp64" else 5e-7  # fp32: float32 stencil products
     assert _rel(out.real, A) < tol and _rel(out.imag, B) < tol
 
 
@@ -179,8 +179,9 @@ def test_bk_raw_matches_oracle(prec, N, step, ncut, nmax):
     # error relative to the typical triangle magnitude (the sums change sign)
     scale = np.max(np.abs(ref))
     assert np.max(np.abs(got - ref)) / scale < TOL[prec]
-    j0 = ncut // step
-    assert _rel(p0.cpu().numpy()[j0 - 1 if j0 > 0 else 0:], p0k[j0 - 1 if j0 > 0 else 0:]) < 1e-10
+    # p0k by Parseval from the half field (float32 in the fp32/bf16x3 pipelines)
+    j0 = max(ncut // step, 1)
+    assert _rel(p0.cpu().numpy()[j0 - 1:], p0k[j0 - 1:]) < (1e-12 if prec == "fp64" else 1e-6)
 
 
 def test_bk_raw_is_deterministic():
