@@ -14,6 +14,12 @@
 //   PSX_PREC_BF16X3 v_mfma_f32_16x16x32_bf16 on split operands
 //                   x = hi + lo (hi = bf16(x), lo = bf16(x - hi)):
 //                   P D ~ Ph Dh + Ph Dl + Pl Dh, float32 accumulators
+//
+// LDS per chunk: the shell values of KC points, rows padded by 4 elements
+// (16 B) so row starts stay 16-byte aligned and consecutive rows fall in
+// different bank groups.  The split-BF16 mode also stores the B operand
+// pre-split (hi and lo planes), so every wave reads its B fragments with one
+// 16-byte LDS load per plane instead of re-splitting them.
 #include "psx_ctx.h"
 
 namespace psx {
@@ -21,6 +27,7 @@ namespace psx {
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 
 constexpr int CT_WAVES = 8;
 constexpr int CT_THREADS = CT_WAVES * 64;
@@ -37,9 +44,25 @@ template <> struct CtTraits<PSX_PREC_F32> {
 };
 template <> struct CtTraits<PSX_PREC_BF16X3> {
   typedef float TD; typedef f4v Acc; typedef float TP;
-  static constexpr int KSTEP = 32, KC = 256, MAXPT = 8;
+  static constexpr int KSTEP = 32, KC = 128, MAXPT = 8;
 };
 
+template <int MODE> __host__ __device__ constexpr int ct_ldw() { return CtTraits<MODE>::KC + 4; }
+// bf16 plane row length (elements): KC + 8 keeps 16-byte alignment
+template <int MODE> __host__ __device__ constexpr int ct_ldwh() { return CtTraits<MODE>::KC + 8; }
+
+template <int MODE>
+static size_t ct_lds_bytes(int ns) {
+  size_t b = (size_t)ns * ct_ldw<MODE>() * sizeof(typename CtTraits<MODE>::TD);
+  if (MODE == PSX_PREC_BF16X3) b += (size_t)2 * ns * ct_ldwh<MODE>() * sizeof(unsigned short);
+  return b;
+}
+
+__device__ __forceinline__ unsigned short bf16_bits(__bf16 h) {
+  return __builtin_bit_cast(unsigned short, h);
+}
+
+// x = hi + lo with hi = bf16(x) (round to nearest even), lo = bf16(x - hi)
 __device__ __forceinline__ void split_bf16(const float (&x)[8], bf8v& hi, bf8v& lo) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -60,23 +83,40 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
   typedef typename Tr::TD TD;
   typedef typename Tr::Acc Acc;
   constexpr int KC = Tr::KC, KS = Tr::KSTEP, MAXPT = Tr::MAXPT;
-  constexpr int LDW = KC + 4;  // padded row length (keeps 16 B alignment)
+  constexpr int LDW = ct_ldw<MODE>();
+  constexpr int LDWH = ct_ldwh<MODE>();
+  constexpr bool BF = MODE == PSX_PREC_BF16X3;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TD* lds = reinterpret_cast<TD*>(smem_raw);
+  unsigned short* ldh = reinterpret_cast<unsigned short*>(smem_raw + (size_t)ns * LDW * sizeof(TD));
+  unsigned short* ldl = ldh + (size_t)ns * LDWH;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int r16 = lane & 15;   // row (A) / column (B) within a 16-block
   const int kq = lane >> 4;    // k group
   const int gw = blockIdx.y * CT_WAVES + wave;
+  constexpr int KL = BF ? 8 : 1;  // consecutive k per lane
 
-  int pt[MAXPT], msk[MAXPT], si[MAXPT], sj[MAXPT];
+  int pt[MAXPT], msk[MAXPT], oi[MAXPT], oj[MAXPT];
+  int need = 0;  // l tiles this wave reads
 #pragma unroll
   for (int q = 0; q < MAXPT; ++q) {
     pt[q] = wave_pt[gw * MAXPT + q];
     msk[q] = wave_mask[gw * MAXPT + q];
-    si[q] = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2] : -1;
-    sj[q] = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2 + 1] : -1;
+    need |= msk[q];
+    const int si = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2] : -1;
+    const int sj = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2 + 1] : -1;
+    // padded rows (si < 0) read shell 0 and are multiplied by zero below
+    oi[q] = (si >= 0 ? si : 0) * LDW + kq * KL;
+    oj[q] = (sj >= 0 ? sj : 0) * LDW + kq * KL;
+    if (si < 0) oj[q] = -1;
+  }
+  int ob[CT_MAXLT];
+#pragma unroll
+  for (int t = 0; t < CT_MAXLT; ++t) {
+    const int s = t * 16 + r16;
+    ob[t] = (s < ns) ? s : -1;
   }
   Acc acc[MAXPT][CT_MAXLT];
 #pragma unroll
@@ -88,32 +128,86 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
 
   const int64_t x_begin = (int64_t)blockIdx.x * pts_per_block;
   const int64_t x_end = min(G, x_begin + pts_per_block);
-  for (int64_t x0 = x_begin; x0 < x_end; x0 += KC) {
-    const int kc = (int)min((int64_t)KC, x_end - x0);
-    // stage D[s][x0 .. x0+KC) for every shell (zero past the end)
-    for (int idx = threadIdx.x; idx < ns * KC; idx += CT_THREADS) {
-      const int s = idx / KC, k = idx - s * KC;
-      lds[s * LDW + k] = k < kc ? D[(int64_t)s * G + x0 + k] : TD(0);
+  // Chunks are prefetched into registers one chunk ahead, so the HBM loads of
+  // chunk c+1 are in flight while chunk c is contracted.  Each thread owns up
+  // to NV groups of 4 consecutive values (ns <= 64 shells).
+  constexpr int NG = KC / 4;                 // 4-value groups per shell row
+  constexpr int NV = 64 * NG / CT_THREADS;   // groups per thread at ns = 64
+  TD pre[NV][4];
+  auto fetch = [&](int64_t xc) {
+    const int kc = (int)min((int64_t)KC, x_end - xc);
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      const int idx = threadIdx.x + u * CT_THREADS;
+      if (idx < ns * NG) {
+        const int s = idx / NG, k = (idx - s * NG) * 4;
+        const TD* src = D + (int64_t)s * G + xc + k;
+        if (k + 4 <= kc) {
+          if constexpr (sizeof(TD) == 4) {
+            const f4v t4 = *reinterpret_cast<const f4v*>(src);
+            pre[u][0] = t4[0]; pre[u][1] = t4[1]; pre[u][2] = t4[2]; pre[u][3] = t4[3];
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) pre[u][e] = src[e];
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) pre[u][e] = (k + e < kc) ? src[e] : TD(0);
+        }
+      }
     }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      const int idx = threadIdx.x + u * CT_THREADS;
+      if (idx < ns * NG) {
+        const int s = idx / NG, k = (idx - s * NG) * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) lds[s * LDW + k + e] = pre[u][e];
+        if constexpr (BF) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const __bf16 h = (__bf16)(float)pre[u][e];
+            ldh[s * LDWH + k + e] = bf16_bits(h);
+            ldl[s * LDWH + k + e] = bf16_bits((__bf16)((float)pre[u][e] - (float)h));
+          }
+        }
+      }
+    }
+  };
+  if (x_begin < x_end) fetch(x_begin);
+  for (int64_t x0 = x_begin; x0 < x_end; x0 += KC) {
+    commit();
     __syncthreads();
+    if (x0 + KC < x_end) fetch(x0 + KC);
     for (int k0 = 0; k0 < KC; k0 += KS) {
-      if constexpr (MODE == PSX_PREC_BF16X3) {
+      if constexpr (BF) {
         bf8v bh[CT_MAXLT], bl[CT_MAXLT];
 #pragma unroll
         for (int t = 0; t < CT_MAXLT; ++t) {
-          float xb[8];
-          const int s = t * 16 + r16;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) xb[e] = (t < nlt && s < ns) ? lds[s * LDW + k0 + kq * 8 + e] : 0.0f;
-          split_bf16(xb, bh[t], bl[t]);
+          if ((need >> t) & 1) {
+            const int s = ob[t] >= 0 ? ob[t] : 0;
+            us8v h = *reinterpret_cast<const us8v*>(ldh + s * LDWH + k0 + kq * 8);
+            us8v l = *reinterpret_cast<const us8v*>(ldl + s * LDWH + k0 + kq * 8);
+            if (ob[t] < 0) { h = 0; l = 0; }
+            bh[t] = __builtin_bit_cast(bf8v, h);
+            bl[t] = __builtin_bit_cast(bf8v, l);
+          }
         }
 #pragma unroll
         for (int q = 0; q < MAXPT; ++q) {
           if (pt[q] < 0) continue;
+          const f4v* pa = reinterpret_cast<const f4v*>(lds + oi[q] + k0);
+          const f4v* pb = reinterpret_cast<const f4v*>(lds + (oj[q] >= 0 ? oj[q] : oi[q]) + k0);
+          const f4v a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+          const float z = oj[q] >= 0 ? 1.0f : 0.0f;
           float xa[8];
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
-            xa[e] = si[q] >= 0 ? lds[si[q] * LDW + k0 + kq * 8 + e] * lds[sj[q] * LDW + k0 + kq * 8 + e] : 0.0f;
+          for (int e = 0; e < 4; ++e) {
+            xa[e] = a0[e] * b0[e] * z;
+            xa[4 + e] = a1[e] * b1[e] * z;
+          }
           bf8v ah, al;
           split_bf16(xa, ah, al);
 #pragma unroll
@@ -127,14 +221,12 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
       } else {
         TD b[CT_MAXLT];
 #pragma unroll
-        for (int t = 0; t < CT_MAXLT; ++t) {
-          const int s = t * 16 + r16;
-          b[t] = (t < nlt && s < ns) ? lds[s * LDW + k0 + kq] : TD(0);
-        }
+        for (int t = 0; t < CT_MAXLT; ++t)
+          b[t] = (((need >> t) & 1) && ob[t] >= 0) ? lds[ob[t] * LDW + k0 + kq] : TD(0);
 #pragma unroll
         for (int q = 0; q < MAXPT; ++q) {
           if (pt[q] < 0) continue;
-          const TD a = si[q] >= 0 ? lds[si[q] * LDW + k0 + kq] * lds[sj[q] * LDW + k0 + kq] : TD(0);
+          const TD a = oj[q] >= 0 ? lds[oi[q] + k0] * lds[oj[q] + k0] : TD(0);
 #pragma unroll
           for (int t = 0; t < CT_MAXLT; ++t) {
             if (!((msk[q] >> t) & 1)) continue;
@@ -162,20 +254,26 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
   }
 }
 
-// b_raw[t] = sum over workgroups (fixed order, float64) of the triangle's entry.
-// tri_loc[t] = slot*256 + lane*4 + reg, with (row, col) -> lane = col + 16*(row/4),
-// reg = row % 4 (16x16 MFMA output layout).
+// Cross-workgroup reduction in float64, in a fixed order (deterministic):
+// stage 1 sums every (group, slot, entry) over the workgroups with coalesced
+// reads; stage 2 gathers each triangle's entry, tri_loc[t] = slot*256 +
+// lane*4 + reg of its (row, col) in the MFMA output layout.
 template <typename TP>
 __global__ void k_contract_reduce(const TP* __restrict__ partials, int64_t nwg, int nslot,
+                                  double* __restrict__ red) {
+  const int slot = blockIdx.x, g = blockIdx.y, e = threadIdx.x;  // 256 threads
+  const TP* p = partials + (int64_t)g * nwg * nslot * 256 + (int64_t)slot * 256 + e;
+  double s = 0.0;
+  for (int64_t w = 0; w < nwg; ++w) s += (double)p[w * nslot * 256];
+  red[((int64_t)g * nslot + slot) * 256 + e] = s;
+}
+
+__global__ void k_contract_gather(const double* __restrict__ red, int nslot,
                                   const int* __restrict__ tri_loc, const int* __restrict__ tri_grp,
                                   int ntri, double* __restrict__ braw) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntri) return;
-  const int loc = tri_loc[t];
-  const int64_t g0 = (int64_t)tri_grp[t] * nwg;
-  double s = 0.0;
-  for (int64_t w = 0; w < nwg; ++w) s += (double)partials[(g0 + w) * nslot * 256 + loc];
-  braw[t] = s;
+  braw[t] = red[(int64_t)tri_grp[t] * nslot * 256 + tri_loc[t]];
 }
 
 template <int MODE>
@@ -196,7 +294,7 @@ static int contract_t(psx_ctx* ctx, const void* D, int N, int step, int ncut, in
   void* part = nullptr;
   PSX_TRY(ctx_buffer(ctx, "contract_partials",
                      (size_t)ngroups * nwg * nslot * 256 * sizeof(typename Tr::TP), &part));
-  const size_t lds = (size_t)cp->ns * (Tr::KC + 4) * sizeof(typename Tr::TD);
+  const size_t lds = ct_lds_bytes<MODE>(cp->ns);
   if (lds > 65536)
     PSX_HIP(hipFuncSetAttribute((const void*)k_contract<MODE>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -205,9 +303,14 @@ static int contract_t(psx_ctx* ctx, const void* D, int N, int step, int ncut, in
                      cp->d_wave_mask, cp->d_wave_slot, cp->d_pair_s, pts_per_block, nslot,
                      (typename Tr::TP*)part);
   PSX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_contract_reduce<typename Tr::TP>, dim3((unsigned)cdiv(cp->ntri, 128)),
-                     dim3(128), 0, ctx->stream, (const typename Tr::TP*)part, nwg, nslot,
-                     cp->d_tri_loc, cp->d_tri_grp, cp->ntri, braw);
+  void* red = nullptr;
+  PSX_TRY(ctx_buffer(ctx, "contract_red", (size_t)ngroups * nslot * 256 * sizeof(double), &red));
+  hipLaunchKernelGGL(k_contract_reduce<typename Tr::TP>, dim3((unsigned)nslot, ngroups), dim3(256),
+                     0, ctx->stream, (const typename Tr::TP*)part, nwg, nslot, (double*)red);
+  PSX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_contract_gather, dim3((unsigned)cdiv(cp->ntri, 128)), dim3(128), 0,
+                     ctx->stream, (const double*)red, nslot, cp->d_tri_loc, cp->d_tri_grp,
+                     cp->ntri, braw);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }

@@ -1,13 +1,19 @@
 // LDS-tiled mixed-radix Stockham FFT engine for gfx950.
 //
-// A workgroup holds a tile of up to TILE_ELEMS complex values (several lines
-// of length N, line-major in LDS) and runs every radix stage in LDS: each
-// stage gathers its butterflies into registers, synchronises, and scatters
-// the autosorted result (Stockham, so no bit reversal pass).  Radices 8, 5,
-// 4, 3, 2 cover every N = 2^a 3^b 5^c up to 1024.
+// A workgroup holds a tile of lines of length N (line-major in LDS, row
+// stride N+1) and runs every radix stage in LDS: each stage gathers its
+// butterflies into registers, synchronises, and scatters the autosorted
+// result (Stockham, so there is no bit-reversal pass).  Radices 8, 5, 4, 3, 2
+// cover every N = 2^a 3^b 5^c.
+//
+// Two instantiations of the same stage code:
+//  * compile-time N (template parameter, the sizes the estimator uses): the
+//    radix sequence, butterfly counts and all index divisions are constants,
+//    and per-thread register arrays are sized exactly;
+//  * runtime N (N = 0): any supported size, used for the others.
 //
 // Twiddles come from a per-N table tw[t] = exp(-2 pi i t / N) computed in
-// float64 on the host (rounded once to float for the FP32 engine); the
+// long double on the host (rounded once to the working precision); the
 // backward sign uses its conjugate.
 #pragma once
 #include "psx_common.h"
@@ -22,6 +28,27 @@ struct FftShape {
   int nstages;
   int radix[FFT_MAX_STAGES];
 };
+
+// Factor N into the engine's radices (8 first, then 5, 4, 3, 2).
+__host__ __device__ constexpr FftShape fft_shape_of(int N) {
+  FftShape sh{N, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int n = N;
+  const int order[5] = {8, 5, 4, 3, 2};
+  for (int i = 0; i < 5; ++i) {
+    const int r = order[i];
+    while (n > 1 && n % r == 0 && sh.nstages < FFT_MAX_STAGES) {
+      sh.radix[sh.nstages++] = r;
+      n /= r;
+    }
+  }
+  if (n != 1) sh.nstages = -1;
+  return sh;
+}
+
+inline bool fft_factor(int N, FftShape* sh) {
+  *sh = fft_shape_of(N);
+  return N >= 2 && sh->nstages > 0;
+}
 
 // ------------------------------------------------------------ small DFTs --
 // In-place DFT of length R on v[0..R), sign S of the exponent.
@@ -68,11 +95,10 @@ template <int S, typename T> __device__ __forceinline__ void dft5(cplx<T>* v) {
 
 template <int S, typename T> __device__ __forceinline__ void dft8(cplx<T>* v) {
   const T h = T(0.70710678118654752440);
-  // radix-2 step between halves, then twiddles w8^k, then two DFT4
+  // radix-2 step between halves, twiddles w8^k, then two DFT4
   cplx<T> a[4], b[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) { a[k] = v[k] + v[k + 4]; b[k] = v[k] - v[k + 4]; }
-  // b[k] *= exp(S i 2 pi k / 8)
   b[1] = mk((b[1].re - T(S) * b[1].im) * h, (b[1].im + T(S) * b[1].re) * h);
   b[2] = mul_i<S>(b[2]);
   b[3] = mk((-b[3].re - T(S) * b[3].im) * h, (-b[3].im + T(S) * b[3].re) * h);
@@ -93,10 +119,10 @@ template <int R, int S, typename T> __device__ __forceinline__ void dftR(cplx<T>
 // ----------------------------------------------------------- one stage ---
 // lds: nlines lines, line l at lds + l*lstride.  Stage with radix R after the
 // first Ns points have been combined (Ns = product of earlier radices).
-template <int R, int S, typename T, int TILE_ELEMS>
-__device__ __forceinline__ void fft_stage(cplx<T>* lds, int N, int Ns, int nlines, int lstride,
+// MAXQ = butterflies per thread for the largest tile.
+template <int R, int S, typename T, int MAXQ, typename IntN, typename IntNs>
+__device__ __forceinline__ void fft_stage(cplx<T>* lds, IntN N, IntNs Ns, int nlines, int lstride,
                                           const cplx<T>* __restrict__ tw) {
-  constexpr int MAXQ = (TILE_ELEMS / R + FFT_THREADS - 1) / FFT_THREADS;
   const int nb = N / R;
   const int total = nb * nlines;
   const int tstride = N / (Ns * R);
@@ -137,38 +163,48 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, int N, int Ns, int nline
   __syncthreads();
 }
 
-// Full transform of every line in the tile (in place in LDS).
-template <int S, typename T, int TILE_ELEMS>
-__device__ __forceinline__ void fft_tile(cplx<T>* lds, const FftShape& sh, int nlines, int lstride,
-                                         const cplx<T>* __restrict__ tw) {
-  int Ns = 1;
-  for (int s = 0; s < sh.nstages; ++s) {
-    const int R = sh.radix[s];
-    switch (R) {
-      case 8: fft_stage<8, S, T, TILE_ELEMS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-      case 5: fft_stage<5, S, T, TILE_ELEMS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-      case 4: fft_stage<4, S, T, TILE_ELEMS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-      case 3: fft_stage<3, S, T, TILE_ELEMS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-      default: fft_stage<2, S, T, TILE_ELEMS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-    }
-    Ns *= R;
+template <int V> struct IntC {
+  static constexpr int value = V;
+  __device__ constexpr operator int() const { return V; }
+};
+
+// Compile-time N: stages unrolled by recursion over the radix sequence.
+template <int N, int STAGE, int NS, int S, typename T, int MAXLINES>
+__device__ __forceinline__ void fft_stages_ct(cplx<T>* lds, int nlines, int lstride,
+                                              const cplx<T>* __restrict__ tw) {
+  constexpr FftShape sh = fft_shape_of(N);
+  if constexpr (STAGE < sh.nstages) {
+    constexpr int R = sh.radix[STAGE];
+    constexpr int MAXQ = (MAXLINES * (N / R) + FFT_THREADS - 1) / FFT_THREADS;
+    fft_stage<R, S, T, MAXQ>(lds, IntC<N>(), IntC<NS>(), nlines, lstride, tw);
+    fft_stages_ct<N, STAGE + 1, NS * R, S, T, MAXLINES>(lds, nlines, lstride, tw);
   }
 }
 
-// Host: factor N into the engine's radices (8 first, then 5, 4, 3, 2).
-inline bool fft_factor(int N, FftShape* sh) {
-  sh->N = N;
-  sh->nstages = 0;
-  int n = N;
-  const int order[5] = {8, 5, 4, 3, 2};
-  for (int r : order) {
-    while (n % r == 0) {
-      if (sh->nstages >= FFT_MAX_STAGES) return false;
-      sh->radix[sh->nstages++] = r;
-      n /= r;
+// Transform every line of the tile in place.  N > 0: compile-time length;
+// N == 0: runtime length sh.N with at most MAXLINES*sh.N <= TILE_ELEMS.
+template <int N, int S, typename T, int MAXLINES, int TILE_ELEMS>
+__device__ __forceinline__ void fft_tile(cplx<T>* lds, const FftShape& sh, int nlines, int lstride,
+                                         const cplx<T>* __restrict__ tw) {
+  if constexpr (N > 0) {
+    fft_stages_ct<N, 0, 1, S, T, MAXLINES>(lds, nlines, lstride, tw);
+  } else {
+    int Ns = 1;
+    for (int s = 0; s < sh.nstages; ++s) {
+      const int R = sh.radix[s];
+      switch (R) {
+        case 8: fft_stage<8, S, T, (TILE_ELEMS / 8 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
+        case 5: fft_stage<5, S, T, (TILE_ELEMS / 5 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
+        case 4: fft_stage<4, S, T, (TILE_ELEMS / 4 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
+        case 3: fft_stage<3, S, T, (TILE_ELEMS / 3 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
+        default: fft_stage<2, S, T, (TILE_ELEMS / 2 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
+      }
+      Ns *= R;
     }
   }
-  return n == 1 && N >= 2;
 }
+
+// Sizes with a compile-time instantiation; others run the runtime-N engine.
+#define PSX_FFT_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(360) X(384) X(512)
 
 }  // namespace psx

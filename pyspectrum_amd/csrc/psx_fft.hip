@@ -9,36 +9,58 @@
 //    over kx <= kmax_j columns, and an x pass that builds two Hermitian lines
 //    per complex FFT (their real and imaginary parts are the two real outputs).
 //
+// Every kernel is instantiated for the compile-time sizes of PSX_FFT_SIZES
+// (N > 0) and once for a runtime N (N = 0).
+//
 // Grid layout everywhere: index (z*N + y)*N + x, x fastest; the half field is
 // (z*N + y)*(N/2+1) + x.
 #include "psx_ctx.h"
 
 namespace psx {
 
-// ------------------------------------------------------------------ passes --
+// Lines per tile: up to 16 (FP32) / 8 (FP64) lines, at most ELEMS complex values.
 template <typename T> struct TileCfg;
 template <> struct TileCfg<float> { static constexpr int ELEMS = 8192; static constexpr int LINES = 16; };
 template <> struct TileCfg<double> { static constexpr int ELEMS = 4096; static constexpr int LINES = 8; };
 
+__host__ __device__ constexpr int tile_lines_for(int N, int elems, int lines) {
+  return elems / N < lines ? elems / N : lines;
+}
+template <typename T, int N> struct Tile {
+  // compile-time N: exact; runtime N: the bound
+  static constexpr int MAXLINES = N > 0 ? tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES)
+                                        : TileCfg<T>::LINES;
+};
+
 template <typename T>
-__device__ __forceinline__ int tile_lines(int N) {
-  int tl = TileCfg<T>::ELEMS / N;
-  return tl < TileCfg<T>::LINES ? tl : TileCfg<T>::LINES;
+static int tile_lines_host(int N) {
+  return tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES);
 }
 
+template <int NC> __device__ __forceinline__ int n_of(const FftShape& sh) {
+  if constexpr (NC > 0) return NC; else return sh.N;
+}
+
+// Copy the twiddle table into LDS (read by every radix stage).
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(cplx<T>* twl, const cplx<T>* __restrict__ tw, int N) {
+  for (int i = threadIdx.x; i < N; i += FFT_THREADS) twl[i] = tw[i];
+}
+
+// ------------------------------------------------------------------ passes --
 // Batched line FFT.  Line (o, c) starts at o*Os + c*Cs, element n at +n*Es.
 // A tile holds tl consecutive inner lines c of one outer index o.
 // FIXED: the input is a pair of int64 fixed-point grids (re, im) scaled by inv_scale.
-template <typename T, int S, bool FIXED>
+template <typename T, int S, bool FIXED, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
     const cplx<T>* in, cplx<T>* out, const int64_t* __restrict__ ga, const int64_t* __restrict__ gb,
     double inv_scale, FftShape sh, const cplx<T>* __restrict__ tw, int64_t Os, int64_t Cs,
     int64_t Es, int n_inner) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int N = sh.N;
+  const int N = n_of<NC>(sh);
   const int lstride = N + 1;
-  const int tl = tile_lines<T>(N);
+  const int tl = NC > 0 ? Tile<T, NC>::MAXLINES : tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES);
   const int ntile = (n_inner + tl - 1) / tl;
   const int64_t o = blockIdx.x / ntile;
   const int c0 = (int)(blockIdx.x - o * ntile) * tl;
@@ -58,21 +80,16 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
     }
     lds[c * lstride + n] = v;
   }
+  cplx<T>* twl = lds + tl * lstride;
+  stage_twiddles(twl, tw, N);
   __syncthreads();
-  fft_tile<S, T, TileCfg<T>::ELEMS>(lds, sh, nl, lstride, tw);
+  fft_tile<NC, S, T, Tile<T, NC>::MAXLINES, TileCfg<T>::ELEMS>(lds, sh, nl, lstride, twl);
   for (int idx = threadIdx.x; idx < total; idx += FFT_THREADS) {
     int c, n;
     if (Es == 1) { c = idx / N; n = idx - c * N; }
     else { n = idx / nl; c = idx - n * nl; }
     out[base + c * Cs + n * Es] = lds[c * lstride + n];
   }
-}
-
-template <typename T>
-static size_t lds_bytes(int N) {
-  int tl = TileCfg<T>::ELEMS / N;
-  if (tl > TileCfg<T>::LINES) tl = TileCfg<T>::LINES;
-  return (size_t)tl * (N + 1) * sizeof(cplx<T>);
 }
 
 template <typename T> static const cplx<T>* tw_of(const FftPlan* p);
@@ -88,26 +105,39 @@ static int set_lds(K kernel, size_t bytes) {
   return PSX_OK;
 }
 
-template <typename T, int S, bool FIXED>
-static int run_pass(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out,
-                    const int64_t* ga, const int64_t* gb, double inv_scale, int axis) {
+template <typename T, int S, bool FIXED, int NC>
+static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out,
+                      const int64_t* ga, const int64_t* gb, double inv_scale, int axis) {
   const int N = p->shape.N;
   const int64_t NN = (int64_t)N * N;
   int64_t Os, Cs, Es;
   if (axis == 0) { Os = NN; Cs = N; Es = 1; }        // x lines: inner y, outer z
   else if (axis == 1) { Os = NN; Cs = 1; Es = N; }   // y lines: inner x, outer z
   else { Os = N; Cs = 1; Es = NN; }                  // z lines: inner x, outer y
-  int tl = TileCfg<T>::ELEMS / N;
-  if (tl > TileCfg<T>::LINES) tl = TileCfg<T>::LINES;
+  const int tl = tile_lines_host<T>(N);
   const int64_t nblocks = (int64_t)N * cdiv(N, tl);
-  const size_t lds = lds_bytes<T>(N);
-  auto kern = k_fft_lines<T, S, FIXED>;
+  const size_t lds = ((size_t)tl * (N + 1) + N) * sizeof(cplx<T>);
+  auto kern = k_fft_lines<T, S, FIXED, NC>;
   PSX_TRY(set_lds(kern, lds));
   hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(FFT_THREADS), lds, ctx->stream, in, out,
                      ga, gb, inv_scale, p->shape, tw_of<T>(p), Os, Cs, Es, N);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
+
+#define PSX_CASE_N(n)                                                                        \
+  case n:                                                                                    \
+    return run_pass_n<T, S, FIXED, n>(ctx, p, in, out, ga, gb, inv_scale, axis);
+
+template <typename T, int S, bool FIXED>
+static int run_pass(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out,
+                    const int64_t* ga, const int64_t* gb, double inv_scale, int axis) {
+  switch (p->shape.N) {
+    PSX_FFT_SIZES(PSX_CASE_N)
+    default: return run_pass_n<T, S, FIXED, 0>(ctx, p, in, out, ga, gb, inv_scale, axis);
+  }
+}
+#undef PSX_CASE_N
 
 template <typename T, int S>
 static int fft3d_t(psx_ctx* ctx, cplx<T>* g, int N) {
@@ -156,8 +186,8 @@ struct ShellDev {
 
 __device__ __forceinline__ int signed_f(int n, int N) { return n <= N / 2 ? n : n - N; }
 
-// z pass: tile (s, ky, kx0): 16 kx columns of one ky row, masked to shell j.
-template <typename T>
+// z pass: tile (s, ky, kx0): NL kx columns of one ky row, masked to shell j.
+template <typename T, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restrict__ H,
                                                          cplx<T>* __restrict__ T1,
                                                          const Tile3* __restrict__ tiles,
@@ -165,7 +195,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
                                                          const cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int N = sh.N, Nh = sd.Nh, lstride = N + 1;
+  const int N = n_of<NC>(sh), Nh = sd.Nh, lstride = N + 1;
   const Tile3 t = tiles[blockIdx.x];
   const int s = t.s, ky = t.a, kx0 = t.b;
   const int j = sd.meta[4 * s], kxw = sd.meta[4 * s + 2];
@@ -182,8 +212,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
     }
     lds[c * lstride + n] = v;
   }
+  cplx<T>* twl = lds + NL * lstride;
+  stage_twiddles(twl, tw, N);
   __syncthreads();
-  fft_tile<-1, T, TileCfg<T>::ELEMS>(lds, sh, NL, lstride, tw);
+  fft_tile<NC, -1, T, NL, NL * 1024>(lds, sh, NL, lstride, twl);
   cplx<T>* dst = T1 + sd.off[s] + (int64_t)ky * kxw + kx0;
   for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
     const int n = idx / NL, c = idx - n * NL;
@@ -191,9 +223,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
   }
 }
 
-// y pass: tile (s, z, kx0): 16 kx columns at one z; rows (ky, kx-tile) that
+// y pass: tile (s, z, kx0): NL kx columns at one z; rows (ky, kx-tile) that
 // the z pass skipped are zero.
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restrict__ T1,
                                                          cplx<T>* __restrict__ T2,
                                                          const Tile3* __restrict__ tiles,
@@ -201,7 +233,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
                                                          const cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int N = sh.N, lstride = N + 1;
+  const int N = n_of<NC>(sh), lstride = N + 1;
   const Tile3 t = tiles[blockIdx.x];
   const int s = t.s, z = t.a, kx0 = t.b;
   const int kxw = sd.meta[4 * s + 2];
@@ -215,8 +247,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
     if ((int64_t)kx0 * kx0 + (int64_t)kys * kys <= mm) v = T1[base + (int64_t)n * kxw + c];
     lds[c * lstride + n] = v;
   }
+  cplx<T>* twl = lds + NL * lstride;
+  stage_twiddles(twl, tw, N);
   __syncthreads();
-  fft_tile<-1, T, TileCfg<T>::ELEMS>(lds, sh, NL, lstride, tw);
+  fft_tile<NC, -1, T, NL, NL * 1024>(lds, sh, NL, lstride, twl);
   for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
     const int n = idx / NL, c = idx - n * NL;
     T2[base + (int64_t)n * kxw + c] = lds[c * lstride + n];
@@ -227,15 +261,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
 // each pair is one complex FFT of X1 + i X2 whose real/imag parts are the two
 // real rows.  kx = 0 and kx = N/2 keep only their real part, which makes the
 // result exactly Re FFT of the Hermitian extension.
-template <typename T, typename TO>
+template <typename T, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restrict__ T2,
-                                                         TO* __restrict__ D,
+                                                         T* __restrict__ D,
                                                          const Tile3* __restrict__ tiles,
                                                          ShellDev sd, FftShape sh,
                                                          const cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int N = sh.N, lstride = N + 1, hN = N / 2;
+  const int N = n_of<NC>(sh), lstride = N + 1, hN = N / 2;
   const Tile3 t = tiles[blockIdx.x];
   const int s = t.s, z = t.a, y0 = t.b;
   const int kxw = sd.meta[4 * s + 2];
@@ -245,6 +279,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
     const int p = idx / N, n = idx - p * N;
     lds[p * lstride + n] = mk(T(0), T(0));
   }
+  cplx<T>* twl = lds + 8 * lstride;
+  stage_twiddles(twl, tw, N);
   __syncthreads();
   const int64_t base = sd.off[s] + (int64_t)z * N * kxw;
   for (int idx = threadIdx.x; idx < npair * kxl; idx += FFT_THREADS) {
@@ -261,16 +297,48 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
     }
   }
   __syncthreads();
-  fft_tile<-1, T, TileCfg<T>::ELEMS>(lds, sh, npair, lstride, tw);
+  fft_tile<NC, -1, T, 8, 8 * 1024>(lds, sh, npair, lstride, twl);
   const int64_t G = (int64_t)N * N * N;
-  TO* dst = D + (int64_t)s * G + ((int64_t)z * N + y0) * N;
+  T* dst = D + (int64_t)s * G + ((int64_t)z * N + y0) * N;
   for (int idx = threadIdx.x; idx < npair * N; idx += FFT_THREADS) {
     const int p = idx / N, n = idx - p * N;
     const cplx<T> v = lds[p * lstride + n];
-    dst[(int64_t)(2 * p) * N + n] = (TO)v.re;
-    dst[(int64_t)(2 * p + 1) * N + n] = (TO)v.im;
+    dst[(int64_t)(2 * p) * N + n] = v.re;
+    dst[(int64_t)(2 * p + 1) * N + n] = v.im;
   }
 }
+
+template <typename T, int NC>
+static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, const cplx<T>* H,
+                          int step, cplx<T>* t1, cplx<T>* t2, T* dout) {
+  const int N = p->shape.N;
+  ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
+  const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
+  const size_t ldsx = ((size_t)8 * (N + 1) + N) * sizeof(cplx<T>);
+  const cplx<T>* tw = tw_of<T>(p);
+  if (sp->nz_tiles > 0) {
+    PSX_TRY(set_lds(k_shell_z<T, NC>, ldsz));
+    hipLaunchKernelGGL((k_shell_z<T, NC>), dim3(sp->nz_tiles), dim3(FFT_THREADS), ldsz, ctx->stream,
+                       H, t1, sp->d_ztiles, sd, p->shape, tw);
+    PSX_CHECK_LAUNCH();
+  }
+  if (sp->ny_tiles > 0) {
+    PSX_TRY(set_lds(k_shell_y<T, NC>, ldsz));
+    hipLaunchKernelGGL((k_shell_y<T, NC>), dim3(sp->ny_tiles), dim3(FFT_THREADS), ldsz, ctx->stream,
+                       (const cplx<T>*)t1, t2, sp->d_ytiles, sd, p->shape, tw);
+    PSX_CHECK_LAUNCH();
+  }
+  if (sp->nx_tiles > 0) {
+    PSX_TRY(set_lds(k_shell_x<T, NC>, ldsx));
+    hipLaunchKernelGGL((k_shell_x<T, NC>), dim3(sp->nx_tiles), dim3(FFT_THREADS), ldsx, ctx->stream,
+                       (const cplx<T>*)t2, dout, sp->d_xtiles, sd, p->shape, tw);
+    PSX_CHECK_LAUNCH();
+  }
+  return PSX_OK;
+}
+
+#define PSX_CASE_N(n) \
+  case n: return shell_fields_n<T, n>(ctx, p, sp, H, step, (cplx<T>*)t1, (cplx<T>*)t2, (T*)dout);
 
 template <typename T>
 static int shell_fields_t(psx_ctx* ctx, const cplx<T>* H, int N, int step, int ncut, int nmax,
@@ -284,30 +352,12 @@ static int shell_fields_t(psx_ctx* ctx, const cplx<T>* H, int N, int step, int n
                      (size_t)sp->t_elems * sizeof(cplx<T>), &t1));
   PSX_TRY(ctx_buffer(ctx, sizeof(T) == 4 ? "shell_t2_f32" : "shell_t2_f64",
                      (size_t)sp->t_elems * sizeof(cplx<T>), &t2));
-  ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
-  const size_t lds16 = (size_t)TileCfg<T>::LINES * (N + 1) * sizeof(cplx<T>);
-  const size_t lds8 = (size_t)8 * (N + 1) * sizeof(cplx<T>);
-  const cplx<T>* tw = tw_of<T>(p);
-  if (sp->nz_tiles > 0) {
-    PSX_TRY(set_lds(k_shell_z<T>, lds16));
-    hipLaunchKernelGGL(k_shell_z<T>, dim3(sp->nz_tiles), dim3(FFT_THREADS), lds16, ctx->stream, H,
-                       (cplx<T>*)t1, sp->d_ztiles, sd, p->shape, tw);
-    PSX_CHECK_LAUNCH();
+  switch (N) {
+    PSX_FFT_SIZES(PSX_CASE_N)
+    default: return shell_fields_n<T, 0>(ctx, p, sp, H, step, (cplx<T>*)t1, (cplx<T>*)t2, (T*)dout);
   }
-  if (sp->ny_tiles > 0) {
-    PSX_TRY(set_lds(k_shell_y<T>, lds16));
-    hipLaunchKernelGGL(k_shell_y<T>, dim3(sp->ny_tiles), dim3(FFT_THREADS), lds16, ctx->stream,
-                       (const cplx<T>*)t1, (cplx<T>*)t2, sp->d_ytiles, sd, p->shape, tw);
-    PSX_CHECK_LAUNCH();
-  }
-  if (sp->nx_tiles > 0) {
-    PSX_TRY(set_lds(k_shell_x<T, T>, lds8));
-    hipLaunchKernelGGL((k_shell_x<T, T>), dim3(sp->nx_tiles), dim3(FFT_THREADS), lds8, ctx->stream,
-                       (const cplx<T>*)t2, (T*)dout, sp->d_xtiles, sd, p->shape, tw);
-    PSX_CHECK_LAUNCH();
-  }
-  return PSX_OK;
 }
+#undef PSX_CASE_N
 
 int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                         int prec, void* dout) {

@@ -1,6 +1,10 @@
 // Mesh front end of the periodic-box estimator: PCS assignment with
 // interlacing, the interlacing/deconvolution combination, and the half <->
 // full field conversions.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include "psx_ctx.h"
 
 namespace psx {
@@ -49,24 +53,40 @@ template <typename W> __device__ __forceinline__ W wmul(W a, W b);
 template <> __device__ __forceinline__ float wmul<float>(float a, float b) { return a * b; }
 template <> __device__ __forceinline__ double wmul<double>(double a, double b) { return a * b; }
 
-// One thread per particle; 2 x 64 contributions are added as int64 fixed
-// point (scale 2^e), so the grid is independent of the order of the adds.
+__device__ __forceinline__ float clip_pos(float v, float clip_max) {
+  // np.clip to [0, clip_max]; NaN is sent to 0 so no cell index leaves the grid
+  v = (v >= 0.0f) ? v : 0.0f;
+  return (v <= clip_max) ? v : clip_max;
+}
+
+// Sort key of a particle: its grid-A base cell in memory order (z, y, x).
+__global__ void k_cell_keys(const float* __restrict__ xyz, int64_t np, int N, float kf_ks,
+                            float clip_max, unsigned* __restrict__ keys, unsigned* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) pcs_coord(clip_pos(xyz[(int64_t)d * np + i], clip_max), kf_ks, false, N, &c[d]);
+  keys[i] = ((unsigned)c[2] * (unsigned)N + (unsigned)c[1]) * (unsigned)N + (unsigned)c[0];
+  idx[i] = (unsigned)i;
+}
+
+// One thread per particle, particles visited in cell order (perm) so that
+// the scattered adds of neighbouring threads share cache lines; the 2 x 64
+// contributions are added as int64 fixed point (scale 2^e), so the grid is
+// independent of the order of the adds.
 template <typename W>
 __global__ __launch_bounds__(256) void k_assign(const float* __restrict__ xyz, const float* __restrict__ w,
+                                                const unsigned* __restrict__ perm,
                                                 int64_t np, int N, float kf_ks, float clip_max,
                                                 double scale, unsigned long long* __restrict__ ga,
                                                 unsigned long long* __restrict__ gb) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const int64_t i = perm ? (int64_t)perm[t] : t;
   float p[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    float v = xyz[(int64_t)d * np + i];
-    // np.clip to [0, clip_max]; NaN is sent to 0 so no cell index leaves the grid
-    v = (v >= 0.0f) ? v : 0.0f;
-    v = (v <= clip_max) ? v : clip_max;
-    p[d] = v;
-  }
+  for (int d = 0; d < 3; ++d) p[d] = clip_pos(xyz[(int64_t)d * np + i], clip_max);
   const W we = (W)(w ? w[i] : 1.0f);
 #pragma unroll
   for (int g = 0; g < 2; ++g) {
@@ -123,17 +143,35 @@ int launch_assign(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, in
   PSX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_fill_u64, dim3(2048), dim3(256), 0, ctx->stream, (unsigned long long*)gb, G);
   PSX_CHECK_LAUNCH();
-  if (np > 0) {
-    if (prec == PSX_PREC_F64)
-      hipLaunchKernelGGL(k_assign<double>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
-                         xyz, w, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
-                         (unsigned long long*)gb);
-    else
-      hipLaunchKernelGGL(k_assign<float>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
-                         xyz, w, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
-                         (unsigned long long*)gb);
-    PSX_CHECK_LAUNCH();
-  }
+  if (np <= 0) return PSX_OK;
+  PSX_REQUIRE(np < (int64_t)1 << 32, "more than 2^32 - 1 particles");
+  // order the particles by base cell (rocPRIM radix sort of 32-bit keys)
+  unsigned *kin, *kout, *vin, *perm;
+  PSX_TRY(ctx_buffer(ctx, "sort_keys_in", (size_t)np * 4, (void**)&kin));
+  PSX_TRY(ctx_buffer(ctx, "sort_keys_out", (size_t)np * 4, (void**)&kout));
+  PSX_TRY(ctx_buffer(ctx, "sort_idx_in", (size_t)np * 4, (void**)&vin));
+  PSX_TRY(ctx_buffer(ctx, "sort_idx_out", (size_t)np * 4, (void**)&perm));
+  hipLaunchKernelGGL(k_cell_keys, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz, np,
+                     N, kf_ks, clip_max, kin, vin);
+  PSX_CHECK_LAUNCH();
+  int bits = 1;
+  while (((int64_t)1 << bits) < G) ++bits;
+  size_t tmp_bytes = 0;
+  PSX_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, perm, (size_t)np, 0, bits,
+                                    ctx->stream));
+  void* tmp;
+  PSX_TRY(ctx_buffer(ctx, "sort_tmp", tmp_bytes > 0 ? tmp_bytes : 16, &tmp));
+  PSX_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, perm, (size_t)np, 0, bits,
+                                    ctx->stream));
+  if (prec == PSX_PREC_F64)
+    hipLaunchKernelGGL(k_assign<double>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
+                       xyz, w, perm, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
+                       (unsigned long long*)gb);
+  else
+    hipLaunchKernelGGL(k_assign<float>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
+                       xyz, w, perm, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
+                       (unsigned long long*)gb);
+  PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
 

@@ -284,3 +284,26 @@ def test_bk_periodic_matches_oracle_small(prec):
     ref = S.Bk_periodic(xyz, None, Lbox, N, 3, 3, 10, counts=counts)
     got = P.Bk_periodic(xyz, Lbox=Lbox, Ngrid=N, step=3, Ncut=3, Nmax=10, precision=prec)
     _bk_compare(got, ref, TOL[prec])
+
+
+# ------------------------------------------------ CPU-runnable config -----
+def test_plain_config_matches_oracle():
+    """1e5 uniform points (seed 1) in a 1000 Mpc/h box, Ngrid=96, step=3, Ncut=3,
+    Nmax=30, FP64: shells 29-30 lie beyond the grid (empty) and 828 of the 2825
+    triangles have N_tri = 0 (zero-count quirk of pyspectrum.py:423-445)."""
+    _say("plain config C1 fp64")
+    rng = np.random.default_rng(1)
+    xyz = rng.uniform(0, 1000.0, size=(3, 100000)).astype(np.float32)
+    N, Lbox, nmax = 96, 1000.0, 30
+    counts = S.counts_Bk123(N, nmax, 3, 3)
+    counts[np.rint(counts / N ** 3) == 0] = 0.0
+    I, J, Lx, _ = S.triangles(nmax, 3, 3)
+    assert int(np.sum(counts[I - 1, J - 1, Lx - 1] == 0)) == 828
+    ref = S.Bk_periodic(xyz, None, Lbox, N, 3, 3, nmax, counts=counts)
+    got = P.Bk_periodic(xyz, Lbox=Lbox, Ngrid=N, step=3, Ncut=3, Nmax=nmax, precision="fp64")
+    _bk_compare(got, ref, 1e-9)
+    assert len(got["i_k1"]) == 2825 - 828 and len(got["b123"]) == 2825
+    pk_ref = S.Pk_periodic(xyz, None, Lbox, N)
+    pk = P.Pk_periodic(xyz, Lbox=Lbox, Ngrid=N, precision="fp64")
+    assert np.array_equal(pk["counts"], pk_ref["counts"])
+    assert _rel(pk["p0k"], pk_ref["p0k"]) < 1e-10
