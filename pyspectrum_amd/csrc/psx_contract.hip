@@ -51,10 +51,12 @@ template <int MODE> __host__ __device__ constexpr int ct_ldw() { return CtTraits
 // bf16 plane row length (elements): KC + 8 keeps 16-byte alignment
 template <int MODE> __host__ __device__ constexpr int ct_ldwh() { return CtTraits<MODE>::KC + 8; }
 
+// ns shell rows plus one all-zero row (index ns) that padded pair rows and
+// padded shell columns point at.
 template <int MODE>
 static size_t ct_lds_bytes(int ns) {
-  size_t b = (size_t)ns * ct_ldw<MODE>() * sizeof(typename CtTraits<MODE>::TD);
-  if (MODE == PSX_PREC_BF16X3) b += (size_t)2 * ns * ct_ldwh<MODE>() * sizeof(unsigned short);
+  size_t b = (size_t)(ns + 1) * ct_ldw<MODE>() * sizeof(typename CtTraits<MODE>::TD);
+  if (MODE == PSX_PREC_BF16X3) b += (size_t)2 * (ns + 1) * ct_ldwh<MODE>() * sizeof(unsigned short);
   return b;
 }
 
@@ -88,8 +90,15 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
   constexpr bool BF = MODE == PSX_PREC_BF16X3;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TD* lds = reinterpret_cast<TD*>(smem_raw);
-  unsigned short* ldh = reinterpret_cast<unsigned short*>(smem_raw + (size_t)ns * LDW * sizeof(TD));
-  unsigned short* ldl = ldh + (size_t)ns * LDWH;
+  unsigned short* ldh = reinterpret_cast<unsigned short*>(smem_raw + (size_t)(ns + 1) * LDW * sizeof(TD));
+  unsigned short* ldl = ldh + (size_t)(ns + 1) * LDWH;
+  for (int k = threadIdx.x; k < LDW; k += CT_THREADS) lds[ns * LDW + k] = TD(0);
+  if constexpr (MODE == PSX_PREC_BF16X3) {
+    for (int k = threadIdx.x; k < LDWH; k += CT_THREADS) {
+      ldh[ns * LDWH + k] = 0;
+      ldl[ns * LDWH + k] = 0;
+    }
+  }
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -107,16 +116,15 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
     need |= msk[q];
     const int si = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2] : -1;
     const int sj = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2 + 1] : -1;
-    // padded rows (si < 0) read shell 0 and are multiplied by zero below
-    oi[q] = (si >= 0 ? si : 0) * LDW + kq * KL;
-    oj[q] = (sj >= 0 ? sj : 0) * LDW + kq * KL;
-    if (si < 0) oj[q] = -1;
+    // padded rows read the zero row
+    oi[q] = (si >= 0 ? si : ns) * LDW + kq * KL;
+    oj[q] = (sj >= 0 ? sj : ns) * LDW + kq * KL;
   }
-  int ob[CT_MAXLT];
+  int ob[CT_MAXLT];  // B rows (shells), padded columns read the zero row
 #pragma unroll
   for (int t = 0; t < CT_MAXLT; ++t) {
     const int s = t * 16 + r16;
-    ob[t] = (s < ns) ? s : -1;
+    ob[t] = (s < ns) ? s : ns;
   }
   Acc acc[MAXPT][CT_MAXLT];
 #pragma unroll
@@ -187,10 +195,8 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
 #pragma unroll
         for (int t = 0; t < CT_MAXLT; ++t) {
           if ((need >> t) & 1) {
-            const int s = ob[t] >= 0 ? ob[t] : 0;
-            us8v h = *reinterpret_cast<const us8v*>(ldh + s * LDWH + k0 + kq * 8);
-            us8v l = *reinterpret_cast<const us8v*>(ldl + s * LDWH + k0 + kq * 8);
-            if (ob[t] < 0) { h = 0; l = 0; }
+            const us8v h = *reinterpret_cast<const us8v*>(ldh + ob[t] * LDWH + k0 + kq * 8);
+            const us8v l = *reinterpret_cast<const us8v*>(ldl + ob[t] * LDWH + k0 + kq * 8);
             bh[t] = __builtin_bit_cast(bf8v, h);
             bl[t] = __builtin_bit_cast(bf8v, l);
           }
@@ -199,14 +205,13 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
         for (int q = 0; q < MAXPT; ++q) {
           if (pt[q] < 0) continue;
           const f4v* pa = reinterpret_cast<const f4v*>(lds + oi[q] + k0);
-          const f4v* pb = reinterpret_cast<const f4v*>(lds + (oj[q] >= 0 ? oj[q] : oi[q]) + k0);
+          const f4v* pb = reinterpret_cast<const f4v*>(lds + oj[q] + k0);
           const f4v a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
-          const float z = oj[q] >= 0 ? 1.0f : 0.0f;
           float xa[8];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            xa[e] = a0[e] * b0[e] * z;
-            xa[4 + e] = a1[e] * b1[e] * z;
+            xa[e] = a0[e] * b0[e];
+            xa[4 + e] = a1[e] * b1[e];
           }
           bf8v ah, al;
           split_bf16(xa, ah, al);
@@ -222,11 +227,11 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
         TD b[CT_MAXLT];
 #pragma unroll
         for (int t = 0; t < CT_MAXLT; ++t)
-          b[t] = (((need >> t) & 1) && ob[t] >= 0) ? lds[ob[t] * LDW + k0 + kq] : TD(0);
+          b[t] = ((need >> t) & 1) ? lds[ob[t] * LDW + k0 + kq] : TD(0);
 #pragma unroll
         for (int q = 0; q < MAXPT; ++q) {
           if (pt[q] < 0) continue;
-          const TD a = oj[q] >= 0 ? lds[oi[q] + k0] * lds[oj[q] + k0] : TD(0);
+          const TD a = lds[oi[q] + k0] * lds[oj[q] + k0];
 #pragma unroll
           for (int t = 0; t < CT_MAXLT; ++t) {
             if (!((msk[q] >> t) & 1)) continue;

@@ -66,12 +66,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
   const int c0 = (int)(blockIdx.x - o * ntile) * tl;
   const int nl = min(tl, n_inner - c0);
   const int64_t base = o * Os + (int64_t)c0 * Cs;
-  const int total = nl * N;
-  for (int idx = threadIdx.x; idx < total; idx += FFT_THREADS) {
+  // offsets within a tile fit in 32 bits for N <= 1024; tl is a constant for
+  // compile-time N, so the index divisions below are multiplications
+  const int cs = (int)Cs, es = (int)Es;
+  for (int idx = threadIdx.x; idx < tl * N; idx += FFT_THREADS) {
     int c, n;
-    if (Es == 1) { c = idx / N; n = idx - c * N; }
-    else { n = idx / nl; c = idx - n * nl; }
-    const int64_t g = base + c * Cs + n * Es;
+    if (es == 1) { c = idx / N; n = idx - c * N; }
+    else { n = idx / tl; c = idx - n * tl; }
+    if (c >= nl) continue;
+    const int64_t g = base + (c * cs + n * es);
     cplx<T> v;
     if constexpr (FIXED) {
       v = mk((T)((double)ga[g] * inv_scale), (T)((double)gb[g] * inv_scale));
@@ -84,11 +87,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
   stage_twiddles(twl, tw, N);
   __syncthreads();
   fft_tile<NC, S, T, Tile<T, NC>::MAXLINES, TileCfg<T>::ELEMS>(lds, sh, nl, lstride, twl);
-  for (int idx = threadIdx.x; idx < total; idx += FFT_THREADS) {
+  for (int idx = threadIdx.x; idx < tl * N; idx += FFT_THREADS) {
     int c, n;
-    if (Es == 1) { c = idx / N; n = idx - c * N; }
-    else { n = idx / nl; c = idx - n * nl; }
-    out[base + c * Cs + n * Es] = lds[c * lstride + n];
+    if (es == 1) { c = idx / N; n = idx - c * N; }
+    else { n = idx / tl; c = idx - n * tl; }
+    if (c >= nl) continue;
+    out[base + (c * cs + n * es)] = lds[c * lstride + n];
   }
 }
 

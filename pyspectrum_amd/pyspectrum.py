@@ -36,6 +36,7 @@ Inputs may be numpy arrays or torch tensors; torch CUDA tensors are used in
 place (no host round trip).
 """
 import os
+import threading
 
 import numpy as np
 import torch
@@ -219,10 +220,11 @@ def Pk_periodic_rsd(xyz, w=None, Lbox=2600, Ngrid=360, rsd=2, Nmubin=10, fft='py
             'mu_kmu': mu_kmu, 'p_kmu': p_kmu - 1. / nbar, 'counts_kmu': n_kmu}
 
 
-def _pk_pbox_rsd(half, irsd, Lbox, Nbin, Nmu, Ngrid, prec, device):
+def _pk_pbox_rsd(half, irsd, Lbox, Nbin, Nmu, Ngrid, prec, device, ctx=None):
     """pk_pbox_rsd on a device half field; the final divisions as estimator.f:246-263."""
+    ctx = ctx or L.context(device)
     out = torch.empty(5 * Nbin + 4 * Nbin * Nmu, dtype=torch.float64, device=_device(device))
-    L.call("psx_pk_pbox_rsd", L.context(device).handle, L.ptr(half), int(irsd), int(Lbox), Nbin,
+    L.call("psx_pk_pbox_rsd", ctx.handle, L.ptr(half), int(irsd), int(Lbox), Nbin,
            Nmu, Ngrid, L.ptr(out), prec)
     o = out.cpu().numpy()
     k, p0, p2, p4, nk = [o[i * Nbin:(i + 1) * Nbin].copy() for i in range(5)]
@@ -249,6 +251,10 @@ def cache_dir():
     d = os.environ.get("PYSPECTRUM_AMD_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "pyspectrum_amd"))
     return d
+
+
+_COUNTS_MEM = {}
+_COUNTS_LOCK = threading.Lock()
 
 
 def _counts_name(Ngrid, Nmax, Ncut, step, fft='pyfftw'):
@@ -284,19 +290,26 @@ def _counts_Bk123(Ngrid=360, Nmax=40, Ncut=3, step=3, fft='pyfftw', silent=True,
     fields and written to ``cache_dir()``.
     """
     name = _counts_name(Ngrid, Nmax, Ncut, step, fft)
-    for d in (dat_dir(), cache_dir()):
-        path = os.path.join(d, name)
-        if os.path.isfile(path):
-            return _read_fortran_record(path, Nmax ** 3).reshape(Nmax, Nmax, Nmax)
-    if not silent:
-        print("--- calculating %s ---" % name)
-    counts = _counts_on_gpu(Ngrid, Nmax, Ncut, step, device)
-    try:
-        os.makedirs(cache_dir(), exist_ok=True)
-        _write_fortran_record(os.path.join(cache_dir(), name), counts)
-    except OSError:
-        pass  # a read-only cache only costs a recomputation
-    return counts
+    with _COUNTS_LOCK:
+        hit = _COUNTS_MEM.get(name)
+        if hit is not None:
+            return hit
+        for d in (dat_dir(), cache_dir()):
+            path = os.path.join(d, name)
+            if os.path.isfile(path):
+                counts = _read_fortran_record(path, Nmax ** 3).reshape(Nmax, Nmax, Nmax)
+                _COUNTS_MEM[name] = counts
+                return counts
+        if not silent:
+            print("--- calculating %s ---" % name)
+        counts = _counts_on_gpu(Ngrid, Nmax, Ncut, step, device)
+        _COUNTS_MEM[name] = counts
+        try:
+            os.makedirs(cache_dir(), exist_ok=True)
+            _write_fortran_record(os.path.join(cache_dir(), name), counts)
+        except OSError:
+            pass  # a read-only cache only costs a recomputation
+        return counts
 
 
 def _counts_on_gpu(Ngrid, Nmax, Ncut, step, device=0):

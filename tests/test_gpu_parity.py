@@ -299,6 +299,14 @@ def test_plain_config_matches_oracle():
     counts[np.rint(counts / N ** 3) == 0] = 0.0
     I, J, Lx, _ = S.triangles(nmax, 3, 3)
     assert int(np.sum(counts[I - 1, J - 1, Lx - 1] == 0)) == 828
+    # the GPU's FP64 counts give the same N_tri; B is then compared on equal counts
+    gpu_counts = P._counts_Bk123(Ngrid=N, Nmax=nmax, Ncut=3, step=3)
+    t_ref = counts[I - 1, J - 1, Lx - 1] / N ** 3
+    t_gpu = gpu_counts[I - 1, J - 1, Lx - 1] / N ** 3
+    assert np.array_equal(np.rint(t_ref), np.rint(t_gpu))
+    # float64 round-off of sums over N^3 cells: relative to max(1, N_tri)
+    assert np.max(np.abs(t_ref - t_gpu) / np.maximum(1.0, np.abs(t_ref))) < 1e-9
+    counts = np.where(counts == 0, 0.0, gpu_counts)
     ref = S.Bk_periodic(xyz, None, Lbox, N, 3, 3, nmax, counts=counts)
     got = P.Bk_periodic(xyz, Lbox=Lbox, Ngrid=N, step=3, Ncut=3, Nmax=nmax, precision="fp64")
     _bk_compare(got, ref, 1e-9)
