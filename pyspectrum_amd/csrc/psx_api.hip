@@ -9,9 +9,6 @@
 namespace psx {
 
 // stage launchers from the other translation units
-int fixed_point_exponent(double wsum_abs);
-int launch_assign(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N, float kf_ks,
-                  float clip_max, double scale, int64_t* ga, int64_t* gb, int prec);
 int launch_fcomb(psx_ctx* ctx, const void* F, void* Hh, int N, double sumw, int prec,
                  const void* rec, const float* Wf, const double* Wd);
 int launch_herm_half(psx_ctx* ctx, const void* F, int in_prec, int64_t sx, int64_t sy, int64_t sz,
@@ -24,6 +21,7 @@ int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int
 int launch_contract(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax, int prec,
                     double* braw);
 int contract_maxpt(int prec);
+int contract_waves_per_group();
 int launch_bk_finalize(psx_ctx* ctx, const double* braw, const double* p0k, const double* cnt,
                        const int* tri, int ntri, double G, double nbar, double a3, double b3,
                        double a6, double b6, double* out);
@@ -105,13 +103,15 @@ int get_fft_plan(psx_ctx* ctx, int N, const FftPlan** out) {
   if (it != ctx->fft_plans.end()) { *out = &it->second; return PSX_OK; }
   FftPlan p;
   if (!fft_factor(N, &p.shape)) return set_error(PSX_ERR_UNSUPPORTED, "unsupported FFT length");
-  std::vector<c64> t64(N);
+  std::vector<c64> full(N), t64(N);
   std::vector<c32> t32(N);
   for (int t = 0; t < N; ++t) {
     const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)N;
-    t64[t] = mk((double)cosl(a), (double)sinl(a));
-    t32[t] = mk((float)t64[t].re, (float)t64[t].im);
+    full[t] = mk((double)cosl(a), (double)sinl(a));
   }
+  // stage-major rows of the FFT engine (psx_fft.h)
+  fft_stage_twiddles(p.shape, full.data(), t64.data(), mk(1.0, 0.0));
+  for (int t = 0; t < N; ++t) t32[t] = mk((float)t64[t].re, (float)t64[t].im);
   PSX_TRY(upload(ctx, t64, (c64**)&p.tw64));
   PSX_TRY(upload(ctx, t32, (c32**)&p.tw32));
   const int Nh = N / 2 + 1;
@@ -256,9 +256,9 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
   }
   cp->nblk = 0;
   for (int m : ptmask) cp->nblk += __builtin_popcount(m);
-  // assign pair tiles to waves: groups of 8 waves x maxpt slots, greedy by load
+  // assign pair tiles to waves: groups of WPG waves x maxpt slots, greedy by load
   const int maxpt = contract_maxpt(prec);
-  const int WPG = 8;
+  const int WPG = contract_waves_per_group();
   const int ngroups = (int)cdiv(cp->npt, WPG * maxpt);
   const int nw = ngroups * WPG;
   std::vector<int> order(cp->npt);
@@ -277,7 +277,9 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
     wave_mask[best * maxpt + cnt[best]] = ptmask[pt];
     pt_wave[pt] = best; pt_q[pt] = cnt[best];
     cnt[best]++;
-    load[best] += __builtin_popcount(ptmask[pt]);
+    // cost of a tile: forming and splitting its products (about three
+    // blocks' worth of MFMA time) plus its output blocks
+    load[best] += 3 + __builtin_popcount(ptmask[pt]);
   }
   // output slots per group
   std::vector<int> wave_slot(nw * maxpt * 4, -1);
@@ -328,15 +330,14 @@ static int fft_periodic_impl(psx_ctx* ctx, const float* xyz, const float* w, int
                              double lbox, double sumw, double wsum_abs, void* half, int prec) {
   const int fp = fft_prec(prec);
   const int64_t G = (int64_t)N * N * N;
-  void *ga, *gb, *grid;
-  PSX_TRY(ctx_buffer(ctx, "assign_a", G * 8, &ga));
-  PSX_TRY(ctx_buffer(ctx, "assign_b", G * 8, &gb));
+  void* grid;
   PSX_TRY(ctx_buffer(ctx, "fft_grid", G * cbytes(fp), &grid));
   const float kf_ks = (float)((double)N / lbox);           // np.float32(float(Ngrid)/Lbox)
   const float clip_max = (float)(lbox * (1.0 - 1e-6));     // np.clip bound, cast to float32
   const double scale = ldexp(1.0, fixed_point_exponent(wsum_abs));
-  PSX_TRY(launch_assign(ctx, xyz, w, np, N, kf_ks, clip_max, scale, (int64_t*)ga, (int64_t*)gb, fp));
-  PSX_TRY(launch_fft3d_from_fixed(ctx, (const int64_t*)ga, (const int64_t*)gb, 1.0 / scale, grid, N, fp));
+  // assignment with the x pass of the backward FFT fused, then the y and z passes
+  PSX_TRY(launch_assign_grid(ctx, xyz, w, np, N, kf_ks, clip_max, scale, grid, fp, true));
+  PSX_TRY(launch_fft3d_axes(ctx, grid, N, 1, fp, 1));
   const FftPlan* ft;
   PSX_TRY(get_fft_plan(ctx, N, &ft));
   PSX_TRY(launch_fcomb(ctx, grid, half, N, sumw, fp, ft->rec, ft->Wf, ft->Wd));
@@ -421,14 +422,9 @@ int psx_assign_quad(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, 
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_REQUIRE(np >= 0, "negative particle count");
   PSX_TRY(check_ngrid(ngrid));
-  const int64_t G = (int64_t)ngrid * ngrid * ngrid;
-  void *ga, *gb;
-  PSX_TRY(ctx_buffer(ctx, "assign_a", G * 8, &ga));
-  PSX_TRY(ctx_buffer(ctx, "assign_b", G * 8, &gb));
   const double scale = ldexp(1.0, fixed_point_exponent(wsum_abs));
-  PSX_TRY(launch_assign(ctx, xyz, w, np, ngrid, kf_ks, clip_max, scale, (int64_t*)ga, (int64_t*)gb, fft_prec(prec)));
-  return launch_fixed_to_complex(ctx, (const int64_t*)ga, (const int64_t*)gb, 1.0 / scale, grid_out,
-                                 ngrid, fft_prec(prec));
+  return launch_assign_grid(ctx, xyz, w, np, ngrid, kf_ks, clip_max, scale, grid_out,
+                            fft_prec(prec), false);
 }
 
 int psx_fft3d(psx_ctx* ctx, void* grid, int ngrid, int sign, int prec) {

@@ -5,9 +5,11 @@
 // tiles of 16, columns the shells in tiles of 16, and only the 16x16 output
 // blocks that hold a triangle (l in [max(i-j, j0), j]) are computed.  A
 // workgroup streams a contiguous range of grid points through LDS in chunks;
-// each wave owns a fixed set of pair tiles and keeps their accumulators in
-// registers for the whole range.  Per-workgroup partial blocks are reduced
-// over workgroups in float64 in a fixed order (deterministic).
+// each wave owns up to MAXPT pair tiles and keeps their accumulators in
+// registers for the whole range.  The launch has one workgroup per resident
+// slot (occupancy x CUs), so the whole grid is one wave of workgroups.
+// Per-workgroup partial blocks are reduced over workgroups in float64 in a
+// fixed order (deterministic).
 //
 //   PSX_PREC_F64    v_mfma_f64_16x16x4f64, float64 shells and accumulators
 //   PSX_PREC_F32    v_mfma_f32_16x16x4f32, float32 shells and accumulators
@@ -15,44 +17,66 @@
 //                   x = hi + lo (hi = bf16(x), lo = bf16(x - hi)):
 //                   P D ~ Ph Dh + Ph Dl + Pl Dh, float32 accumulators
 //
-// LDS per chunk: the shell values of KC points, rows padded by 4 elements
-// (16 B) so row starts stay 16-byte aligned and consecutive rows fall in
-// different bank groups.  The split-BF16 mode also stores the B operand
-// pre-split (hi and lo planes), so every wave reads its B fragments with one
-// 16-byte LDS load per plane instead of re-splitting them.
+// LDS per chunk: the shell values of KC points (one padded row per shell plus
+// an all-zero row that padded pair rows and shell columns point at).  The
+// split-BF16 mode also stores the B operand pre-split (hi and lo planes), so
+// every wave reads its B fragments with one 16-byte LDS load per plane.
+//
+// K order.  The sum over grid points is order-free, so each mode maps the
+// MFMA k index to points in the way that gives 16-byte LDS reads: BF16X3
+// lane group kq holds 8 consecutive points of a 32-point step; F32/F64 lane
+// group kq owns the quarter [kq*KC/4, (kq+1)*KC/4) of the chunk and walks it
+// one point per MFMA, reading 4 points (4 MFMAs) per vector load.
+#include <algorithm>
+
 #include "psx_ctx.h"
 
 namespace psx {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 
-constexpr int CT_WAVES = 8;
+// Compile-time tuning: waves per workgroup, pair tiles per wave in the
+// split-BF16 mode, and the minimum waves per SIMD that register allocation
+// must allow (1 = no constraint).
+#ifndef PSX_CT_WAVES
+#define PSX_CT_WAVES 8
+#endif
+#ifndef PSX_CT_MAXPT_BF
+#define PSX_CT_MAXPT_BF 4
+#endif
+#ifndef PSX_CT_MINW
+#define PSX_CT_MINW 1
+#endif
+constexpr int CT_WAVES = PSX_CT_WAVES;
 constexpr int CT_THREADS = CT_WAVES * 64;
-constexpr int CT_MAXLT = 4;
+constexpr int CT_MAXLT = 4;   // l tiles in the plan layout (ns <= 64)
 
 template <int MODE> struct CtTraits;
 template <> struct CtTraits<PSX_PREC_F64> {
   typedef double TD; typedef d4v Acc; typedef double TP;
-  static constexpr int KSTEP = 4, KC = 128, MAXPT = 4;
+  static constexpr int KC = 128, MAXPT = 4, PAD = 2;
 };
 template <> struct CtTraits<PSX_PREC_F32> {
   typedef float TD; typedef f4v Acc; typedef float TP;
-  static constexpr int KSTEP = 4, KC = 256, MAXPT = 8;
+  static constexpr int KC = 128, MAXPT = 4, PAD = 4;
 };
 template <> struct CtTraits<PSX_PREC_BF16X3> {
   typedef float TD; typedef f4v Acc; typedef float TP;
-  static constexpr int KSTEP = 32, KC = 128, MAXPT = 8;
+  static constexpr int KC = 128, MAXPT = PSX_CT_MAXPT_BF, PAD = 4;
 };
 
-template <int MODE> __host__ __device__ constexpr int ct_ldw() { return CtTraits<MODE>::KC + 4; }
-// bf16 plane row length (elements): KC + 8 keeps 16-byte alignment
+// Row strides (elements): padded so that 8 consecutive rows start in
+// different 16-byte bank groups and every row start stays 16-byte aligned.
+template <int MODE> __host__ __device__ constexpr int ct_ldw() {
+  return CtTraits<MODE>::KC + CtTraits<MODE>::PAD;
+}
 template <int MODE> __host__ __device__ constexpr int ct_ldwh() { return CtTraits<MODE>::KC + 8; }
 
-// ns shell rows plus one all-zero row (index ns) that padded pair rows and
-// padded shell columns point at.
 template <int MODE>
 static size_t ct_lds_bytes(int ns) {
   size_t b = (size_t)(ns + 1) * ct_ldw<MODE>() * sizeof(typename CtTraits<MODE>::TD);
@@ -60,9 +84,7 @@ static size_t ct_lds_bytes(int ns) {
   return b;
 }
 
-__device__ __forceinline__ unsigned short bf16_bits(__bf16 h) {
-  return __builtin_bit_cast(unsigned short, h);
-}
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // x = hi + lo with hi = bf16(x) (round to nearest even), lo = bf16(x - hi)
 __device__ __forceinline__ void split_bf16(const float (&x)[8], bf8v& hi, bf8v& lo) {
@@ -75,72 +97,75 @@ __device__ __forceinline__ void split_bf16(const float (&x)[8], bf8v& hi, bf8v& 
 }
 
 // D: [ns][G] shell fields; partials: [gridDim.y][gridDim.x][nslot][256].
-template <int MODE>
-__global__ __launch_bounds__(CT_THREADS) void k_contract(
-    const typename CtTraits<MODE>::TD* __restrict__ D, int64_t G, int ns, int nlt,
+// NLT: l tiles read (ns <= 16*NLT).
+template <int MODE, int NLT>
+__global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
+    const typename CtTraits<MODE>::TD* __restrict__ D, int64_t G, int ns,
     const int* __restrict__ wave_pt, const int* __restrict__ wave_mask,
     const int* __restrict__ wave_slot, const int* __restrict__ pair_s, int64_t pts_per_block,
     int nslot, typename CtTraits<MODE>::TP* __restrict__ partials) {
   typedef CtTraits<MODE> Tr;
   typedef typename Tr::TD TD;
   typedef typename Tr::Acc Acc;
-  constexpr int KC = Tr::KC, KS = Tr::KSTEP, MAXPT = Tr::MAXPT;
+  constexpr int KC = Tr::KC, MAXPT = Tr::MAXPT;
   constexpr int LDW = ct_ldw<MODE>();
   constexpr int LDWH = ct_ldwh<MODE>();
   constexpr bool BF = MODE == PSX_PREC_BF16X3;
+  constexpr int KQ = KC / 4;  // F32/F64: points per lane group per chunk
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TD* lds = reinterpret_cast<TD*>(smem_raw);
   unsigned short* ldh = reinterpret_cast<unsigned short*>(smem_raw + (size_t)(ns + 1) * LDW * sizeof(TD));
   unsigned short* ldl = ldh + (size_t)(ns + 1) * LDWH;
   for (int k = threadIdx.x; k < LDW; k += CT_THREADS) lds[ns * LDW + k] = TD(0);
-  if constexpr (MODE == PSX_PREC_BF16X3) {
+  if constexpr (BF) {
     for (int k = threadIdx.x; k < LDWH; k += CT_THREADS) {
       ldh[ns * LDWH + k] = 0;
       ldl[ns * LDWH + k] = 0;
     }
   }
 
-  const int wave = threadIdx.x >> 6;
+  const int wave = uniform((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int r16 = lane & 15;   // row (A) / column (B) within a 16-block
   const int kq = lane >> 4;    // k group
-  const int gw = blockIdx.y * CT_WAVES + wave;
-  constexpr int KL = BF ? 8 : 1;  // consecutive k per lane
+  const int gw = (int)blockIdx.y * CT_WAVES + wave;
+  const int kbase = BF ? kq * 8 : kq * KQ;
 
-  int pt[MAXPT], msk[MAXPT], oi[MAXPT], oj[MAXPT];
-  int need = 0;  // l tiles this wave reads
+  // wave-uniform tile list (pair tiles fill q = 0 .. nq-1)
+  int nq = 0, msk[MAXPT], oi[MAXPT], oj[MAXPT];
 #pragma unroll
   for (int q = 0; q < MAXPT; ++q) {
-    pt[q] = wave_pt[gw * MAXPT + q];
-    msk[q] = wave_mask[gw * MAXPT + q];
-    need |= msk[q];
-    const int si = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2] : -1;
-    const int sj = pt[q] >= 0 ? pair_s[(pt[q] * 16 + r16) * 2 + 1] : -1;
+    const int p = uniform(wave_pt[gw * MAXPT + q]);
+    msk[q] = uniform(wave_mask[gw * MAXPT + q]);
+    nq += p >= 0 ? 1 : 0;
+    const int si = p >= 0 ? pair_s[(p * 16 + r16) * 2] : -1;
+    const int sj = p >= 0 ? pair_s[(p * 16 + r16) * 2 + 1] : -1;
     // padded rows read the zero row
-    oi[q] = (si >= 0 ? si : ns) * LDW + kq * KL;
-    oj[q] = (sj >= 0 ? sj : ns) * LDW + kq * KL;
+    oi[q] = (si >= 0 ? si : ns) * LDW + kbase;
+    oj[q] = (sj >= 0 ? sj : ns) * LDW + kbase;
   }
-  int ob[CT_MAXLT];  // B rows (shells), padded columns read the zero row
+  int ob[NLT], obh[NLT];  // B rows (shells); padded columns read the zero row
 #pragma unroll
-  for (int t = 0; t < CT_MAXLT; ++t) {
+  for (int t = 0; t < NLT; ++t) {
     const int s = t * 16 + r16;
-    ob[t] = (s < ns) ? s : ns;
+    ob[t] = ((s < ns) ? s : ns) * LDW + kbase;
+    obh[t] = ((s < ns) ? s : ns) * LDWH + kbase;
   }
-  Acc acc[MAXPT][CT_MAXLT];
+  Acc acc[MAXPT][NLT];
 #pragma unroll
   for (int q = 0; q < MAXPT; ++q)
 #pragma unroll
-    for (int t = 0; t < CT_MAXLT; ++t)
+    for (int t = 0; t < NLT; ++t)
 #pragma unroll
       for (int e = 0; e < 4; ++e) acc[q][t][e] = 0;
 
   const int64_t x_begin = (int64_t)blockIdx.x * pts_per_block;
   const int64_t x_end = min(G, x_begin + pts_per_block);
   // Chunks are prefetched into registers one chunk ahead, so the HBM loads of
-  // chunk c+1 are in flight while chunk c is contracted.  Each thread owns up
-  // to NV groups of 4 consecutive values (ns <= 64 shells).
-  constexpr int NG = KC / 4;                 // 4-value groups per shell row
-  constexpr int NV = 64 * NG / CT_THREADS;   // groups per thread at ns = 64
+  // chunk c+1 are in flight while chunk c is contracted.  Each thread owns NV
+  // groups of 4 consecutive values (ns <= 16*NLT shells).
+  constexpr int NG = KC / 4;                       // 4-value groups per shell row
+  constexpr int NV = (NLT * 16 * NG + CT_THREADS - 1) / CT_THREADS;
   TD pre[NV][4];
   auto fetch = [&](int64_t xc) {
     const int kc = (int)min((int64_t)KC, x_end - xc);
@@ -155,8 +180,9 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
             const f4v t4 = *reinterpret_cast<const f4v*>(src);
             pre[u][0] = t4[0]; pre[u][1] = t4[1]; pre[u][2] = t4[2]; pre[u][3] = t4[3];
           } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) pre[u][e] = src[e];
+            const d2v t0 = *reinterpret_cast<const d2v*>(src);
+            const d2v t1 = *reinterpret_cast<const d2v*>(src + 2);
+            pre[u][0] = t0[0]; pre[u][1] = t0[1]; pre[u][2] = t1[0]; pre[u][3] = t1[1];
           }
         } else {
 #pragma unroll
@@ -171,15 +197,25 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
       const int idx = threadIdx.x + u * CT_THREADS;
       if (idx < ns * NG) {
         const int s = idx / NG, k = (idx - s * NG) * 4;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) lds[s * LDW + k + e] = pre[u][e];
+        if constexpr (sizeof(TD) == 4) {
+          f4v v;
+          v[0] = pre[u][0]; v[1] = pre[u][1]; v[2] = pre[u][2]; v[3] = pre[u][3];
+          *reinterpret_cast<f4v*>(lds + s * LDW + k) = v;
+        } else {
+          d2v v0, v1;
+          v0[0] = pre[u][0]; v0[1] = pre[u][1]; v1[0] = pre[u][2]; v1[1] = pre[u][3];
+          *reinterpret_cast<d2v*>(lds + s * LDW + k) = v0;
+          *reinterpret_cast<d2v*>(lds + s * LDW + k + 2) = v1;
+        }
         if constexpr (BF) {
+          bf4v h, l;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const __bf16 h = (__bf16)(float)pre[u][e];
-            ldh[s * LDWH + k + e] = bf16_bits(h);
-            ldl[s * LDWH + k + e] = bf16_bits((__bf16)((float)pre[u][e] - (float)h));
+            h[e] = (__bf16)(float)pre[u][e];
+            l[e] = (__bf16)((float)pre[u][e] - (float)h[e]);
           }
+          *reinterpret_cast<bf4v*>(ldh + s * LDWH + k) = h;
+          *reinterpret_cast<bf4v*>(ldl + s * LDWH + k) = l;
         }
       }
     }
@@ -189,21 +225,18 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
     commit();
     __syncthreads();
     if (x0 + KC < x_end) fetch(x0 + KC);
-    for (int k0 = 0; k0 < KC; k0 += KS) {
-      if constexpr (BF) {
-        bf8v bh[CT_MAXLT], bl[CT_MAXLT];
+    if constexpr (BF) {
 #pragma unroll
-        for (int t = 0; t < CT_MAXLT; ++t) {
-          if ((need >> t) & 1) {
-            const us8v h = *reinterpret_cast<const us8v*>(ldh + ob[t] * LDWH + k0 + kq * 8);
-            const us8v l = *reinterpret_cast<const us8v*>(ldl + ob[t] * LDWH + k0 + kq * 8);
-            bh[t] = __builtin_bit_cast(bf8v, h);
-            bl[t] = __builtin_bit_cast(bf8v, l);
-          }
+      for (int k0 = 0; k0 < KC; k0 += 32) {
+        bf8v bh[NLT], bl[NLT];
+#pragma unroll
+        for (int t = 0; t < NLT; ++t) {
+          bh[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const us8v*>(ldh + obh[t] + k0));
+          bl[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const us8v*>(ldl + obh[t] + k0));
         }
 #pragma unroll
         for (int q = 0; q < MAXPT; ++q) {
-          if (pt[q] < 0) continue;
+          if (q >= nq) break;
           const f4v* pa = reinterpret_cast<const f4v*>(lds + oi[q] + k0);
           const f4v* pb = reinterpret_cast<const f4v*>(lds + oj[q] + k0);
           const f4v a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
@@ -216,29 +249,57 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
           bf8v ah, al;
           split_bf16(xa, ah, al);
 #pragma unroll
-          for (int t = 0; t < CT_MAXLT; ++t) {
-            if (!((msk[q] >> t) & 1)) continue;
-            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[q][t], 0, 0, 0);
-            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[q][t], 0, 0, 0);
-            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[q][t], 0, 0, 0);
+          for (int t = 0; t < NLT; ++t) {
+            if ((msk[q] >> t) & 1) {
+              acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[q][t], 0, 0, 0);
+              acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[q][t], 0, 0, 0);
+              acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[q][t], 0, 0, 0);
+            }
           }
         }
-      } else {
-        TD b[CT_MAXLT];
+      }
+    } else {
 #pragma unroll
-        for (int t = 0; t < CT_MAXLT; ++t)
-          b[t] = ((need >> t) & 1) ? lds[ob[t] * LDW + k0 + kq] : TD(0);
+      for (int s0 = 0; s0 < KQ; s0 += 4) {
+        TD b[NLT][4];
+#pragma unroll
+        for (int t = 0; t < NLT; ++t) {
+          if constexpr (sizeof(TD) == 4) {
+            const f4v v = *reinterpret_cast<const f4v*>(lds + ob[t] + s0);
+            b[t][0] = v[0]; b[t][1] = v[1]; b[t][2] = v[2]; b[t][3] = v[3];
+          } else {
+            const d2v v0 = *reinterpret_cast<const d2v*>(lds + ob[t] + s0);
+            const d2v v1 = *reinterpret_cast<const d2v*>(lds + ob[t] + s0 + 2);
+            b[t][0] = v0[0]; b[t][1] = v0[1]; b[t][2] = v1[0]; b[t][3] = v1[1];
+          }
+        }
 #pragma unroll
         for (int q = 0; q < MAXPT; ++q) {
-          if (pt[q] < 0) continue;
-          const TD a = lds[oi[q] + k0] * lds[oj[q] + k0];
+          if (q >= nq) break;
+          TD a[4];
+          if constexpr (sizeof(TD) == 4) {
+            const f4v ai = *reinterpret_cast<const f4v*>(lds + oi[q] + s0);
+            const f4v aj = *reinterpret_cast<const f4v*>(lds + oj[q] + s0);
 #pragma unroll
-          for (int t = 0; t < CT_MAXLT; ++t) {
-            if (!((msk[q] >> t) & 1)) continue;
-            if constexpr (MODE == PSX_PREC_F64)
-              acc[q][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[q][t], 0, 0, 0);
-            else
-              acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[t], acc[q][t], 0, 0, 0);
+            for (int e = 0; e < 4; ++e) a[e] = ai[e] * aj[e];
+          } else {
+            const d2v i0 = *reinterpret_cast<const d2v*>(lds + oi[q] + s0);
+            const d2v i1 = *reinterpret_cast<const d2v*>(lds + oi[q] + s0 + 2);
+            const d2v j0 = *reinterpret_cast<const d2v*>(lds + oj[q] + s0);
+            const d2v j1 = *reinterpret_cast<const d2v*>(lds + oj[q] + s0 + 2);
+            a[0] = i0[0] * j0[0]; a[1] = i0[1] * j0[1]; a[2] = i1[0] * j1[0]; a[3] = i1[1] * j1[1];
+          }
+#pragma unroll
+          for (int t = 0; t < NLT; ++t) {
+            if ((msk[q] >> t) & 1) {
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                if constexpr (MODE == PSX_PREC_F64)
+                  acc[q][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[e], b[t][e], acc[q][t], 0, 0, 0);
+                else
+                  acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[t][e], acc[q][t], 0, 0, 0);
+              }
+            }
           }
         }
       }
@@ -250,8 +311,8 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
 #pragma unroll
   for (int q = 0; q < MAXPT; ++q) {
 #pragma unroll
-    for (int t = 0; t < CT_MAXLT; ++t) {
-      const int slot = wave_slot[(gw * MAXPT + q) * CT_MAXLT + t];
+    for (int t = 0; t < NLT; ++t) {
+      const int slot = uniform(wave_slot[(gw * MAXPT + q) * CT_MAXLT + t]);
       if (slot < 0) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) out[(int64_t)slot * 256 + lane * 4 + e] = acc[q][t][e];
@@ -259,18 +320,36 @@ __global__ __launch_bounds__(CT_THREADS) void k_contract(
   }
 }
 
-// Cross-workgroup reduction in float64, in a fixed order (deterministic):
-// stage 1 sums every (group, slot, entry) over the workgroups with coalesced
-// reads; stage 2 gathers each triangle's entry, tri_loc[t] = slot*256 +
+// Cross-workgroup reduction in float64, in a fixed order (deterministic).
+// Stage 1: block (slot, quarter, group) of 64 entries x 16 segments; thread
+// (e, seg) sums its segment of the workgroups, then segment sums are added in
+// order.  Stage 2 gathers each triangle's entry, tri_loc[t] = slot*256 +
 // lane*4 + reg of its (row, col) in the MFMA output layout.
+constexpr int RED_SEG = 16;
+
 template <typename TP>
-__global__ void k_contract_reduce(const TP* __restrict__ partials, int64_t nwg, int nslot,
-                                  double* __restrict__ red) {
-  const int slot = blockIdx.x, g = blockIdx.y, e = threadIdx.x;  // 256 threads
-  const TP* p = partials + (int64_t)g * nwg * nslot * 256 + (int64_t)slot * 256 + e;
+__global__ __launch_bounds__(64 * RED_SEG) void k_contract_reduce(const TP* __restrict__ partials,
+                                                                  int64_t nwg, int nslot,
+                                                                  double* __restrict__ red) {
+  __shared__ double part[RED_SEG][64];
+  const int slot = blockIdx.x, g = blockIdx.z;
+  const int e = threadIdx.x & 63, seg = threadIdx.x >> 6;
+  const int entry = (int)blockIdx.y * 64 + e;
+  const int64_t per = (nwg + RED_SEG - 1) / RED_SEG;
+  const int64_t w0 = seg * per;
+  const int64_t w1 = min(nwg, w0 + per);
+  const int64_t stride = (int64_t)nslot * 256;
+  const TP* p = partials + ((int64_t)g * nwg * nslot + slot) * 256 + entry;
   double s = 0.0;
-  for (int64_t w = 0; w < nwg; ++w) s += (double)p[w * nslot * 256];
-  red[((int64_t)g * nslot + slot) * 256 + e] = s;
+  for (int64_t w = w0; w < w1; ++w) s += (double)p[w * stride];
+  part[seg][e] = s;
+  __syncthreads();
+  if (seg == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < RED_SEG; ++k) t += part[k][e];
+    red[((int64_t)g * nslot + slot) * 256 + entry] = t;
+  }
 }
 
 __global__ void k_contract_gather(const double* __restrict__ red, int nslot,
@@ -281,17 +360,32 @@ __global__ void k_contract_gather(const double* __restrict__ red, int nslot,
   braw[t] = red[(int64_t)tri_grp[t] * nslot * 256 + tri_loc[t]];
 }
 
-template <int MODE>
-static int contract_t(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax,
-                      double* braw) {
+// Workgroups resident at once on this device for the contraction kernel.
+template <int MODE, int NLT>
+static int contract_slots(psx_ctx* ctx, size_t lds, int* out) {
+  static thread_local std::map<std::tuple<int, int, int, size_t>, int> cache;
+  const auto key = std::make_tuple(ctx->device, MODE, NLT, lds);
+  auto it = cache.find(key);
+  if (it != cache.end()) { *out = it->second; return PSX_OK; }
+  int per_cu = 0, cus = 0;
+  PSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_contract<MODE, NLT>, CT_THREADS, lds));
+  PSX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  if (per_cu < 1) per_cu = 1;
+  *out = cache[key] = per_cu * cus;
+  return PSX_OK;
+}
+
+template <int MODE, int NLT>
+static int contract_t(psx_ctx* ctx, const ContractPlan* cp, const void* D, int N, double* braw) {
   typedef CtTraits<MODE> Tr;
-  const ContractPlan* cp;
-  PSX_TRY(get_contract_plan(ctx, MODE, step, ncut, nmax, &cp));
-  PSX_REQUIRE(cp->maxpt == Tr::MAXPT, "contraction plan built for another precision");
   const int64_t G = (int64_t)N * N * N;
   const int ngroups = cp->waves / CT_WAVES;
+  const size_t lds = ct_lds_bytes<MODE>(cp->ns);
+  PSX_TRY(set_lds(k_contract<MODE, NLT>, lds));
+  int slots = 0;
+  PSX_TRY((contract_slots<MODE, NLT>(ctx, lds, &slots)));
   const int64_t nchunks = cdiv(G, Tr::KC);
-  int64_t nwg = 1024;
+  int64_t nwg = std::max<int64_t>(1, slots / ngroups);
   if (nwg > nchunks) nwg = nchunks;
   const int64_t pts_per_block = cdiv(nchunks, nwg) * Tr::KC;
   nwg = cdiv(G, pts_per_block);
@@ -299,19 +393,16 @@ static int contract_t(psx_ctx* ctx, const void* D, int N, int step, int ncut, in
   void* part = nullptr;
   PSX_TRY(ctx_buffer(ctx, "contract_partials",
                      (size_t)ngroups * nwg * nslot * 256 * sizeof(typename Tr::TP), &part));
-  const size_t lds = ct_lds_bytes<MODE>(cp->ns);
-  if (lds > 65536)
-    PSX_HIP(hipFuncSetAttribute((const void*)k_contract<MODE>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_contract<MODE>, dim3((unsigned)nwg, ngroups), dim3(CT_THREADS), lds,
-                     ctx->stream, (const typename Tr::TD*)D, G, cp->ns, cp->nlt, cp->d_wave_pt,
+  hipLaunchKernelGGL((k_contract<MODE, NLT>), dim3((unsigned)nwg, ngroups), dim3(CT_THREADS), lds,
+                     ctx->stream, (const typename Tr::TD*)D, G, cp->ns, cp->d_wave_pt,
                      cp->d_wave_mask, cp->d_wave_slot, cp->d_pair_s, pts_per_block, nslot,
                      (typename Tr::TP*)part);
   PSX_CHECK_LAUNCH();
   void* red = nullptr;
   PSX_TRY(ctx_buffer(ctx, "contract_red", (size_t)ngroups * nslot * 256 * sizeof(double), &red));
-  hipLaunchKernelGGL(k_contract_reduce<typename Tr::TP>, dim3((unsigned)nslot, ngroups), dim3(256),
-                     0, ctx->stream, (const typename Tr::TP*)part, nwg, nslot, (double*)red);
+  hipLaunchKernelGGL(k_contract_reduce<typename Tr::TP>, dim3((unsigned)nslot, 4, ngroups),
+                     dim3(64 * RED_SEG), 0, ctx->stream, (const typename Tr::TP*)part, nwg, nslot,
+                     (double*)red);
   PSX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_contract_gather, dim3((unsigned)cdiv(cp->ntri, 128)), dim3(128), 0,
                      ctx->stream, (const double*)red, nslot, cp->d_tri_loc, cp->d_tri_grp,
@@ -319,6 +410,23 @@ static int contract_t(psx_ctx* ctx, const void* D, int N, int step, int ncut, in
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
+
+template <int MODE>
+static int contract_mode(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax,
+                         double* braw) {
+  const ContractPlan* cp;
+  PSX_TRY(get_contract_plan(ctx, MODE, step, ncut, nmax, &cp));
+  PSX_REQUIRE(cp->maxpt == CtTraits<MODE>::MAXPT, "contraction plan built for another precision");
+  switch (cp->nlt) {
+    case 1: return contract_t<MODE, 1>(ctx, cp, D, N, braw);
+    case 2: return contract_t<MODE, 2>(ctx, cp, D, N, braw);
+    case 3: return contract_t<MODE, 3>(ctx, cp, D, N, braw);
+    case 4: return contract_t<MODE, 4>(ctx, cp, D, N, braw);
+    default: return set_error(PSX_ERR_UNSUPPORTED, "at most 64 shells (Nmax - Ncut//step + 1 <= 64)");
+  }
+}
+
+int contract_waves_per_group() { return CT_WAVES; }
 
 int contract_maxpt(int prec) {
   if (prec == PSX_PREC_F64) return CtTraits<PSX_PREC_F64>::MAXPT;
@@ -328,9 +436,9 @@ int contract_maxpt(int prec) {
 
 int launch_contract(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax, int prec,
                     double* braw) {
-  if (prec == PSX_PREC_F64) return contract_t<PSX_PREC_F64>(ctx, D, N, step, ncut, nmax, braw);
-  if (prec == PSX_PREC_F32) return contract_t<PSX_PREC_F32>(ctx, D, N, step, ncut, nmax, braw);
-  return contract_t<PSX_PREC_BF16X3>(ctx, D, N, step, ncut, nmax, braw);
+  if (prec == PSX_PREC_F64) return contract_mode<PSX_PREC_F64>(ctx, D, N, step, ncut, nmax, braw);
+  if (prec == PSX_PREC_F32) return contract_mode<PSX_PREC_F32>(ctx, D, N, step, ncut, nmax, braw);
+  return contract_mode<PSX_PREC_BF16X3>(ctx, D, N, step, ncut, nmax, braw);
 }
 
 // ----------------------------------------------------------- finalize -----

@@ -89,13 +89,30 @@ int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, co
 int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
                       const ContractPlan** out);
 
+template <typename T> inline const cplx<T>* plan_tw(const FftPlan* p);
+template <> inline const cplx<float>* plan_tw<float>(const FftPlan* p) { return (const cplx<float>*)p->tw32; }
+template <> inline const cplx<double>* plan_tw<double>(const FftPlan* p) { return (const cplx<double>*)p->tw64; }
+
+// Opt a kernel in to more than 64 KB of dynamic LDS.
+template <typename K>
+inline int set_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) {
+    PSX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+  }
+  return PSX_OK;
+}
+
 // stage launchers (defined in the .hip files)
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec);
-int launch_fft3d_from_fixed(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
-                            void* grid, int N, int prec);
+// line passes along axes first_axis..2 only (0 = x, 1 = y, 2 = z)
+int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis);
+// PCS + interlacing assignment into the complex grid (re = grid A, im = grid B);
+// fft_x: also transform every x line (backward, unnormalised) before the store.
+int launch_assign_grid(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
+                       float kf_ks, float clip_max, double scale, void* grid, int prec, bool fft_x);
+int fixed_point_exponent(double wsum_abs);
 int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                         int prec, void* dout);
-int launch_fixed_to_complex(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
-                            void* grid, int N, int prec);
 int launch_fill_unit(psx_ctx* ctx, double* half, int64_t n);
 }  // namespace psx

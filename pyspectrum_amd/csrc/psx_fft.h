@@ -3,8 +3,16 @@
 // A workgroup holds a tile of lines of length N (line-major in LDS, row
 // stride N+1) and runs every radix stage in LDS: each stage gathers its
 // butterflies into registers, synchronises, and scatters the autosorted
-// result (Stockham, so there is no bit-reversal pass).  Radices 8, 5, 4, 3, 2
-// cover every N = 2^a 3^b 5^c.
+// result (Stockham, so there is no bit-reversal pass).  Radices 9, 8, 5, 4,
+// 3, 2 cover every N = 2^a 3^b 5^c.
+//
+// Thread mapping: a line has TPL = 256/MAXLINES threads, thread (line, lane)
+// owns butterflies j = lane + TPL*q of its line, so every LDS address is a
+// per-thread base plus a compile-time offset; only k = j mod Ns (the twiddle
+// row) is computed per butterfly.
+//
+// Stage order: odd radices first (their stride-R scatter in the first stage
+// touches every LDS bank once per 16 lanes), then 4/2, then the 8s.
 //
 // Two instantiations of the same stage code:
 //  * compile-time N (template parameter, the sizes the estimator uses): the
@@ -12,9 +20,10 @@
 //    and per-thread register arrays are sized exactly;
 //  * runtime N (N = 0): any supported size, used for the others.
 //
-// Twiddles come from a per-N table tw[t] = exp(-2 pi i t / N) computed in
-// long double on the host (rounded once to the working precision); the
-// backward sign uses its conjugate.
+// Twiddles: one table of N entries per N, stage after stage, row k of a
+// stage holding exp(-2 pi i r k / (Ns R)) for r = 1..R-1 (Ns = product of
+// the earlier radices); computed in long double on the host and rounded
+// once to the working precision.  The backward sign uses the conjugate.
 #pragma once
 #include "psx_common.h"
 
@@ -29,25 +38,60 @@ struct FftShape {
   int radix[FFT_MAX_STAGES];
 };
 
-// Factor N into the engine's radices (8 first, then 5, 4, 3, 2).
+// Factor N = 2^a 3^b 5^c into stages: 5s, 9s, a 3, a 4 or 2, then 8s.
 __host__ __device__ constexpr FftShape fft_shape_of(int N) {
   FftShape sh{N, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
-  int n = N;
-  const int order[5] = {8, 5, 4, 3, 2};
-  for (int i = 0; i < 5; ++i) {
-    const int r = order[i];
-    while (n > 1 && n % r == 0 && sh.nstages < FFT_MAX_STAGES) {
-      sh.radix[sh.nstages++] = r;
-      n /= r;
-    }
-  }
-  if (n != 1) sh.nstages = -1;
+  int n = N, c2 = 0, c3 = 0, c5 = 0;
+  while (n > 1 && n % 2 == 0) { n /= 2; ++c2; }
+  while (n > 1 && n % 3 == 0) { n /= 3; ++c3; }
+  while (n > 1 && n % 5 == 0) { n /= 5; ++c5; }
+  if (n != 1) { sh.nstages = -1; return sh; }
+  int r[FFT_MAX_STAGES + 8] = {};
+  int ns = 0;
+  for (int i = 0; i < c5; ++i) r[ns++] = 5;
+  for (int i = 0; i < c3 / 2; ++i) r[ns++] = 9;
+  if (c3 % 2) r[ns++] = 3;
+  if (c2 % 3 == 1) r[ns++] = 2;
+  if (c2 % 3 == 2) r[ns++] = 4;
+  for (int i = 0; i < c2 / 3; ++i) r[ns++] = 8;
+  if (ns > FFT_MAX_STAGES) { sh.nstages = -1; return sh; }
+  for (int i = 0; i < ns; ++i) sh.radix[i] = r[i];
+  sh.nstages = ns;
   return sh;
 }
 
 inline bool fft_factor(int N, FftShape* sh) {
   *sh = fft_shape_of(N);
   return N >= 2 && sh->nstages > 0;
+}
+
+// Lines per tile: up to 16 (FP32) / 8 (FP64) lines, at most ELEMS complex values.
+template <typename T> struct TileCfg;
+template <> struct TileCfg<float> { static constexpr int ELEMS = 8192; static constexpr int LINES = 16; };
+template <> struct TileCfg<double> { static constexpr int ELEMS = 4096; static constexpr int LINES = 8; };
+
+__host__ __device__ constexpr int tile_lines_for(int N, int elems, int lines) {
+  return elems / N < lines ? elems / N : lines;
+}
+template <typename T, int N> struct Tile {
+  // compile-time N: exact; runtime N: the bound
+  static constexpr int MAXLINES = N > 0 ? tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES)
+                                        : TileCfg<T>::LINES;
+};
+
+template <typename T>
+inline int tile_lines_host(int N) {
+  return tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES);
+}
+
+template <int NC> __device__ __forceinline__ int n_of(const FftShape& sh) {
+  if constexpr (NC > 0) return NC; else return sh.N;
+}
+
+// Copy the twiddle table into LDS (read by every radix stage).
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(cplx<T>* twl, const cplx<T>* __restrict__ tw, int N) {
+  for (int i = threadIdx.x; i < N; i += FFT_THREADS) twl[i] = tw[i];
 }
 
 // ------------------------------------------------------------ small DFTs --
@@ -108,41 +152,79 @@ template <int S, typename T> __device__ __forceinline__ void dft8(cplx<T>* v) {
   for (int k = 0; k < 4; ++k) { v[2 * k] = a[k]; v[2 * k + 1] = b[k]; }
 }
 
+// Length 9 as 3 x 3: n = 3 n1 + n2, k = k1 + 3 k2; DFT3 over n1, twiddle
+// W9^(n2 k1), DFT3 over n2.
+template <int S, typename T> __device__ __forceinline__ void dft9(cplx<T>* v) {
+  const T c1 = T(0.76604444311897803520), s1 = T(S) * T(0.64278760968653932632);   // 2 pi/9
+  const T c2 = T(0.17364817766693034885), s2 = T(S) * T(0.98480775301220805936);   // 4 pi/9
+  const T c4 = T(-0.93969262078590838405), s4 = T(S) * T(0.34202014332566873304);  // 8 pi/9
+  cplx<T> a[3][3];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    cplx<T> t[3] = {v[n2], v[3 + n2], v[6 + n2]};
+    dft3<S>(t);
+    a[n2][0] = t[0]; a[n2][1] = t[1]; a[n2][2] = t[2];
+  }
+  a[1][1] = a[1][1] * mk(c1, s1);
+  a[1][2] = a[1][2] * mk(c2, s2);
+  a[2][1] = a[2][1] * mk(c2, s2);
+  a[2][2] = a[2][2] * mk(c4, s4);
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    cplx<T> t[3] = {a[0][k1], a[1][k1], a[2][k1]};
+    dft3<S>(t);
+    v[k1] = t[0]; v[k1 + 3] = t[1]; v[k1 + 6] = t[2];
+  }
+}
+
 template <int R, int S, typename T> __device__ __forceinline__ void dftR(cplx<T>* v) {
   if constexpr (R == 2) dft2<S>(v);
   else if constexpr (R == 3) dft3<S>(v);
   else if constexpr (R == 4) dft4<S>(v);
   else if constexpr (R == 5) dft5<S>(v);
+  else if constexpr (R == 9) dft9<S>(v);
   else dft8<S>(v);
 }
 
 // ----------------------------------------------------------- one stage ---
+template <int V> struct IntC {
+  static constexpr int value = V;
+  __device__ constexpr operator int() const { return V; }
+};
+template <typename I> struct IsOne { static constexpr bool value = false; };
+template <> struct IsOne<IntC<1>> { static constexpr bool value = true; };
+
 // lds: nlines lines, line l at lds + l*lstride.  Stage with radix R after the
-// first Ns points have been combined (Ns = product of earlier radices).
-// MAXQ = butterflies per thread for the largest tile.
-template <int R, int S, typename T, int MAXQ, typename IntN, typename IntNs>
-__device__ __forceinline__ void fft_stage(cplx<T>* lds, IntN N, IntNs Ns, int nlines, int lstride,
-                                          const cplx<T>* __restrict__ tw) {
+// first Ns points have been combined (Ns = product of earlier radices); tw is
+// this stage's twiddle rows [Ns][R-1].  Thread (line, lane), TPL lanes per
+// line, owns butterflies j = lane + TPL*q, q < MAXQ.
+template <int R, int S, typename T, int MAXQ, typename IntTPL, typename IntN, typename IntNs>
+__device__ __forceinline__ void fft_stage(cplx<T>* lds, IntN N, IntNs Ns, IntTPL TPL, int nlines,
+                                          int lstride, const cplx<T>* __restrict__ tw) {
   const int nb = N / R;
-  const int total = nb * nlines;
-  const int tstride = N / (Ns * R);
+  const int line = (int)threadIdx.x / TPL;
+  const int lane = (int)threadIdx.x - line * TPL;
+  const bool act = line < nlines;
+  cplx<T>* base = lds + line * lstride;
   cplx<T> v[MAXQ][R];
+  int kk[MAXQ];
 #pragma unroll
   for (int q = 0; q < MAXQ; ++q) {
-    const int b = threadIdx.x + q * FFT_THREADS;
-    if (b < total) {
-      const int line = b / nb;
-      const int j = b - line * nb;
-      const cplx<T>* src = lds + line * lstride;
-      const int k = j % Ns;
+    const int j = lane + q * TPL;
+    kk[q] = 0;
+    if (act && j < nb) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[j + r * nb];
-      const int t = k * tstride;
+      for (int r = 0; r < R; ++r) v[q][r] = base[j + r * nb];
+      if constexpr (!IsOne<IntNs>::value) {
+        const int k = j % Ns;
+        kk[q] = k;
+        const cplx<T>* w = tw + k * (R - 1);
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        cplx<T> w = tw[r * t];
-        if (S > 0) w.im = -w.im;
-        v[q][r] = v[q][r] * w;
+        for (int r = 1; r < R; ++r) {
+          cplx<T> wr = w[r - 1];
+          if (S > 0) wr.im = -wr.im;
+          v[q][r] = v[q][r] * wr;
+        }
       }
       dftR<R, S>(v[q]);
     }
@@ -150,12 +232,10 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, IntN N, IntNs Ns, int nl
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXQ; ++q) {
-    const int b = threadIdx.x + q * FFT_THREADS;
-    if (b < total) {
-      const int line = b / nb;
-      const int j = b - line * nb;
-      const int k = j % Ns;
-      cplx<T>* dst = lds + line * lstride + (j - k) * R + k;
+    const int j = lane + q * TPL;
+    if (act && j < nb) {
+      const int k = kk[q];
+      cplx<T>* dst = base + (j - k) * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) dst[r * Ns] = v[q][r];
     }
@@ -163,22 +243,26 @@ __device__ __forceinline__ void fft_stage(cplx<T>* lds, IntN N, IntNs Ns, int nl
   __syncthreads();
 }
 
-template <int V> struct IntC {
-  static constexpr int value = V;
-  __device__ constexpr operator int() const { return V; }
-};
-
-// Compile-time N: stages unrolled by recursion over the radix sequence.
-template <int N, int STAGE, int NS, int S, typename T, int MAXLINES>
+// Compile-time N: stages unrolled by recursion over the radix sequence; TW
+// is the offset of the stage's twiddle rows in the table.
+template <int N, int STAGE, int NS, int TW, int S, typename T, int MAXLINES>
 __device__ __forceinline__ void fft_stages_ct(cplx<T>* lds, int nlines, int lstride,
                                               const cplx<T>* __restrict__ tw) {
   constexpr FftShape sh = fft_shape_of(N);
   if constexpr (STAGE < sh.nstages) {
     constexpr int R = sh.radix[STAGE];
-    constexpr int MAXQ = (MAXLINES * (N / R) + FFT_THREADS - 1) / FFT_THREADS;
-    fft_stage<R, S, T, MAXQ>(lds, IntC<N>(), IntC<NS>(), nlines, lstride, tw);
-    fft_stages_ct<N, STAGE + 1, NS * R, S, T, MAXLINES>(lds, nlines, lstride, tw);
+    constexpr int TPL = FFT_THREADS / MAXLINES;
+    constexpr int MAXQ = (N / R + TPL - 1) / TPL;
+    fft_stage<R, S, T, MAXQ>(lds, IntC<N>(), IntC<NS>(), IntC<TPL>(), nlines, lstride, tw + TW);
+    fft_stages_ct<N, STAGE + 1, NS * R, TW + NS * (R - 1), S, T, MAXLINES>(lds, nlines, lstride, tw);
   }
+}
+
+// Butterflies per thread of the runtime-N engine: nb*nlines <= TILE_ELEMS/R
+// and TPL = floor(256/nlines) >= (256 - nlines)/nlines, so nb/TPL <=
+// (TILE_ELEMS/R)/(256 - MAXLINES).
+template <int R, int TILE_ELEMS, int MAXLINES> constexpr int fft_maxq_rt() {
+  return (TILE_ELEMS / R) / (FFT_THREADS - MAXLINES) + 1;
 }
 
 // Transform every line of the tile in place.  N > 0: compile-time length;
@@ -187,21 +271,40 @@ template <int N, int S, typename T, int MAXLINES, int TILE_ELEMS>
 __device__ __forceinline__ void fft_tile(cplx<T>* lds, const FftShape& sh, int nlines, int lstride,
                                          const cplx<T>* __restrict__ tw) {
   if constexpr (N > 0) {
-    fft_stages_ct<N, 0, 1, S, T, MAXLINES>(lds, nlines, lstride, tw);
+    fft_stages_ct<N, 0, 1, 0, S, T, MAXLINES>(lds, nlines, lstride, tw);
   } else {
-    int Ns = 1;
+    const int TPL = FFT_THREADS / (nlines > 0 ? nlines : 1);
+    int Ns = 1, off = 0;
     for (int s = 0; s < sh.nstages; ++s) {
       const int R = sh.radix[s];
+      const cplx<T>* t = tw + off;
       switch (R) {
-        case 8: fft_stage<8, S, T, (TILE_ELEMS / 8 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-        case 5: fft_stage<5, S, T, (TILE_ELEMS / 5 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-        case 4: fft_stage<4, S, T, (TILE_ELEMS / 4 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-        case 3: fft_stage<3, S, T, (TILE_ELEMS / 3 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
-        default: fft_stage<2, S, T, (TILE_ELEMS / 2 + FFT_THREADS - 1) / FFT_THREADS>(lds, sh.N, Ns, nlines, lstride, tw); break;
+        case 9: fft_stage<9, S, T, fft_maxq_rt<9, TILE_ELEMS, MAXLINES>()>(lds, sh.N, Ns, TPL, nlines, lstride, t); break;
+        case 8: fft_stage<8, S, T, fft_maxq_rt<8, TILE_ELEMS, MAXLINES>()>(lds, sh.N, Ns, TPL, nlines, lstride, t); break;
+        case 5: fft_stage<5, S, T, fft_maxq_rt<5, TILE_ELEMS, MAXLINES>()>(lds, sh.N, Ns, TPL, nlines, lstride, t); break;
+        case 4: fft_stage<4, S, T, fft_maxq_rt<4, TILE_ELEMS, MAXLINES>()>(lds, sh.N, Ns, TPL, nlines, lstride, t); break;
+        case 3: fft_stage<3, S, T, fft_maxq_rt<3, TILE_ELEMS, MAXLINES>()>(lds, sh.N, Ns, TPL, nlines, lstride, t); break;
+        default: fft_stage<2, S, T, fft_maxq_rt<2, TILE_ELEMS, MAXLINES>()>(lds, sh.N, Ns, TPL, nlines, lstride, t); break;
       }
+      off += Ns * (R - 1);
       Ns *= R;
     }
   }
+}
+
+// Host: the stage twiddle table of N entries (the last one unused).
+template <typename C>
+inline void fft_stage_twiddles(const FftShape& sh, const C* full /* exp(-2 pi i t/N), t < N */,
+                               C* out, C one) {
+  int Ns = 1, o = 0;
+  for (int s = 0; s < sh.nstages; ++s) {
+    const int R = sh.radix[s];
+    const int stride = sh.N / (Ns * R);
+    for (int k = 0; k < Ns; ++k)
+      for (int r = 1; r < R; ++r) out[o++] = full[r * k * stride];
+    Ns *= R;
+  }
+  while (o < sh.N) out[o++] = one;
 }
 
 // Sizes with a compile-time instantiation; others run the runtime-N engine.

@@ -2,7 +2,8 @@
 //
 //  * Data transform (pyspectrum.py:1060-1080): the packed interlaced grid
 //    A + iB, backward (e^{+i}), unnormalised, as three batched line passes
-//    (x, y, z).  The x pass converts the fixed-point assignment grids on load.
+//    (x, y, z).  In FFT_periodic the x pass runs inside the assignment kernel
+//    (psx_mesh.hip), which holds whole x lines in LDS; y and z follow here.
 //  * Shell fields (pyspectrum.py:387-400): d_j(x) = Re FFT_forward[delta 1_j]
 //    for every shell j, as a pruned Hermitian (c2r) transform of the half
 //    field: a z pass over the (kx, ky) lines that meet the shell, a y pass
@@ -18,44 +19,13 @@
 
 namespace psx {
 
-// Lines per tile: up to 16 (FP32) / 8 (FP64) lines, at most ELEMS complex values.
-template <typename T> struct TileCfg;
-template <> struct TileCfg<float> { static constexpr int ELEMS = 8192; static constexpr int LINES = 16; };
-template <> struct TileCfg<double> { static constexpr int ELEMS = 4096; static constexpr int LINES = 8; };
-
-__host__ __device__ constexpr int tile_lines_for(int N, int elems, int lines) {
-  return elems / N < lines ? elems / N : lines;
-}
-template <typename T, int N> struct Tile {
-  // compile-time N: exact; runtime N: the bound
-  static constexpr int MAXLINES = N > 0 ? tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES)
-                                        : TileCfg<T>::LINES;
-};
-
-template <typename T>
-static int tile_lines_host(int N) {
-  return tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES);
-}
-
-template <int NC> __device__ __forceinline__ int n_of(const FftShape& sh) {
-  if constexpr (NC > 0) return NC; else return sh.N;
-}
-
-// Copy the twiddle table into LDS (read by every radix stage).
-template <typename T>
-__device__ __forceinline__ void stage_twiddles(cplx<T>* twl, const cplx<T>* __restrict__ tw, int N) {
-  for (int i = threadIdx.x; i < N; i += FFT_THREADS) twl[i] = tw[i];
-}
-
 // ------------------------------------------------------------------ passes --
 // Batched line FFT.  Line (o, c) starts at o*Os + c*Cs, element n at +n*Es.
 // A tile holds tl consecutive inner lines c of one outer index o.
-// FIXED: the input is a pair of int64 fixed-point grids (re, im) scaled by inv_scale.
-template <typename T, int S, bool FIXED, int NC>
+template <typename T, int S, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
-    const cplx<T>* in, cplx<T>* out, const int64_t* __restrict__ ga, const int64_t* __restrict__ gb,
-    double inv_scale, FftShape sh, const cplx<T>* __restrict__ tw, int64_t Os, int64_t Cs,
-    int64_t Es, int n_inner) {
+    const cplx<T>* in, cplx<T>* out, FftShape sh, const cplx<T>* __restrict__ tw, int64_t Os,
+    int64_t Cs, int64_t Es, int n_inner) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int N = n_of<NC>(sh);
@@ -74,14 +44,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
     if (es == 1) { c = idx / N; n = idx - c * N; }
     else { n = idx / tl; c = idx - n * tl; }
     if (c >= nl) continue;
-    const int64_t g = base + (c * cs + n * es);
-    cplx<T> v;
-    if constexpr (FIXED) {
-      v = mk((T)((double)ga[g] * inv_scale), (T)((double)gb[g] * inv_scale));
-    } else {
-      v = in[g];
-    }
-    lds[c * lstride + n] = v;
+    lds[c * lstride + n] = in[base + (c * cs + n * es)];
   }
   cplx<T>* twl = lds + tl * lstride;
   stage_twiddles(twl, tw, N);
@@ -96,22 +59,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
   }
 }
 
-template <typename T> static const cplx<T>* tw_of(const FftPlan* p);
-template <> const cplx<float>* tw_of<float>(const FftPlan* p) { return (const cplx<float>*)p->tw32; }
-template <> const cplx<double>* tw_of<double>(const FftPlan* p) { return (const cplx<double>*)p->tw64; }
-
-template <typename K>
-static int set_lds(K kernel, size_t bytes) {
-  if (bytes > 65536) {
-    PSX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)bytes));
-  }
-  return PSX_OK;
-}
-
-template <typename T, int S, bool FIXED, int NC>
-static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out,
-                      const int64_t* ga, const int64_t* gb, double inv_scale, int axis) {
+template <typename T, int S, int NC>
+static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out, int axis) {
   const int N = p->shape.N;
   const int64_t NN = (int64_t)N * N;
   int64_t Os, Cs, Es;
@@ -121,61 +70,46 @@ static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>
   const int tl = tile_lines_host<T>(N);
   const int64_t nblocks = (int64_t)N * cdiv(N, tl);
   const size_t lds = ((size_t)tl * (N + 1) + N) * sizeof(cplx<T>);
-  auto kern = k_fft_lines<T, S, FIXED, NC>;
+  auto kern = k_fft_lines<T, S, NC>;
   PSX_TRY(set_lds(kern, lds));
   hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(FFT_THREADS), lds, ctx->stream, in, out,
-                     ga, gb, inv_scale, p->shape, tw_of<T>(p), Os, Cs, Es, N);
+                     p->shape, plan_tw<T>(p), Os, Cs, Es, N);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
 
 #define PSX_CASE_N(n)                                                                        \
   case n:                                                                                    \
-    return run_pass_n<T, S, FIXED, n>(ctx, p, in, out, ga, gb, inv_scale, axis);
+    return run_pass_n<T, S, n>(ctx, p, in, out, axis);
 
-template <typename T, int S, bool FIXED>
-static int run_pass(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out,
-                    const int64_t* ga, const int64_t* gb, double inv_scale, int axis) {
+template <typename T, int S>
+static int run_pass(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out, int axis) {
   switch (p->shape.N) {
     PSX_FFT_SIZES(PSX_CASE_N)
-    default: return run_pass_n<T, S, FIXED, 0>(ctx, p, in, out, ga, gb, inv_scale, axis);
+    default: return run_pass_n<T, S, 0>(ctx, p, in, out, axis);
   }
 }
 #undef PSX_CASE_N
 
 template <typename T, int S>
-static int fft3d_t(psx_ctx* ctx, cplx<T>* g, int N) {
+static int fft3d_t(psx_ctx* ctx, cplx<T>* g, int N, int first_axis) {
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
-  for (int axis = 0; axis < 3; ++axis)
-    PSX_TRY((run_pass<T, S, false>(ctx, p, g, g, nullptr, nullptr, 0.0, axis)));
+  for (int axis = first_axis; axis < 3; ++axis) PSX_TRY((run_pass<T, S>(ctx, p, g, g, axis)));
   return PSX_OK;
+}
+
+int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis) {
+  if (prec == PSX_PREC_F64) {
+    return sign > 0 ? fft3d_t<double, 1>(ctx, (cplx<double>*)grid, N, first_axis)
+                    : fft3d_t<double, -1>(ctx, (cplx<double>*)grid, N, first_axis);
+  }
+  return sign > 0 ? fft3d_t<float, 1>(ctx, (cplx<float>*)grid, N, first_axis)
+                  : fft3d_t<float, -1>(ctx, (cplx<float>*)grid, N, first_axis);
 }
 
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec) {
-  if (prec == PSX_PREC_F64) {
-    return sign > 0 ? fft3d_t<double, 1>(ctx, (cplx<double>*)grid, N)
-                    : fft3d_t<double, -1>(ctx, (cplx<double>*)grid, N);
-  }
-  return sign > 0 ? fft3d_t<float, 1>(ctx, (cplx<float>*)grid, N)
-                  : fft3d_t<float, -1>(ctx, (cplx<float>*)grid, N);
-}
-
-template <typename T>
-static int fft3d_fixed_t(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
-                         cplx<T>* g, int N) {
-  const FftPlan* p;
-  PSX_TRY(get_fft_plan(ctx, N, &p));
-  PSX_TRY((run_pass<T, 1, true>(ctx, p, nullptr, g, ga, gb, inv_scale, 0)));
-  PSX_TRY((run_pass<T, 1, false>(ctx, p, g, g, nullptr, nullptr, 0.0, 1)));
-  PSX_TRY((run_pass<T, 1, false>(ctx, p, g, g, nullptr, nullptr, 0.0, 2)));
-  return PSX_OK;
-}
-
-int launch_fft3d_from_fixed(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
-                            void* grid, int N, int prec) {
-  if (prec == PSX_PREC_F64) return fft3d_fixed_t<double>(ctx, ga, gb, inv_scale, (cplx<double>*)grid, N);
-  return fft3d_fixed_t<float>(ctx, ga, gb, inv_scale, (cplx<float>*)grid, N);
+  return launch_fft3d_axes(ctx, grid, N, sign, prec, 0);
 }
 
 // ------------------------------------------------------------ shell passes --
@@ -319,7 +253,7 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
   ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
   const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
   const size_t ldsx = ((size_t)8 * (N + 1) + N) * sizeof(cplx<T>);
-  const cplx<T>* tw = tw_of<T>(p);
+  const cplx<T>* tw = plan_tw<T>(p);
   if (sp->nz_tiles > 0) {
     PSX_TRY(set_lds(k_shell_z<T, NC>, ldsz));
     hipLaunchKernelGGL((k_shell_z<T, NC>), dim3(sp->nz_tiles), dim3(FFT_THREADS), ldsz, ctx->stream,

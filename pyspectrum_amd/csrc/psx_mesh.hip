@@ -59,71 +59,210 @@ __device__ __forceinline__ float clip_pos(float v, float clip_max) {
   return (v <= clip_max) ? v : clip_max;
 }
 
-// Sort key of a particle: its grid-A base cell in memory order (z, y, x).
-__global__ void k_cell_keys(const float* __restrict__ xyz, int64_t np, int N, float kf_ks,
-                            float clip_max, unsigned* __restrict__ keys, unsigned* __restrict__ idx) {
+// The grid is assigned in tiles of whole x lines: a tile is a ty x tz block
+// of (y, z) lines, ty*tz no more than the FFT engine's lines per tile.
+// Particles are sorted by the tile of their grid-A base cell; the workgroup
+// of a tile visits the particles of every tile whose stencils can reach it
+// and adds the contributions to its own cells as int64 fixed point (scale
+// 2^e) with LDS atomics -- integer sums, so the grid does not depend on the
+// order of the adds -- then converts to the working precision and stores
+// each line once (in FFT_periodic after the x pass of the FFT).
+struct AssignTiling {
+  int ty, tz, nty, ntz;  // lines per tile along y and z; tiles along y and z
+};
+
+template <typename T>
+static AssignTiling assign_tiling(int N) {
+  const int tl = tile_lines_host<T>(N);
+  AssignTiling a;
+  a.ty = tl >= 4 ? 4 : tl;
+  a.tz = tl / a.ty;
+  a.nty = (N + a.ty - 1) / a.ty;
+  a.ntz = (N + a.tz - 1) / a.tz;
+  return a;
+}
+
+// Sort key of a particle: the assignment tile of its grid-A base cell.
+__global__ void k_tile_keys(const float* __restrict__ xyz, int64_t np, int N, float kf_ks,
+                            float clip_max, AssignTiling at, unsigned* __restrict__ keys,
+                            unsigned* __restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  int c[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) pcs_coord(clip_pos(xyz[(int64_t)d * np + i], clip_max), kf_ks, false, N, &c[d]);
-  keys[i] = ((unsigned)c[2] * (unsigned)N + (unsigned)c[1]) * (unsigned)N + (unsigned)c[0];
+  int cy, cz;
+  pcs_coord(clip_pos(xyz[np + i], clip_max), kf_ks, false, N, &cy);
+  pcs_coord(clip_pos(xyz[2 * np + i], clip_max), kf_ks, false, N, &cz);
+  keys[i] = (unsigned)(cz / at.tz) * (unsigned)at.nty + (unsigned)(cy / at.ty);
   idx[i] = (unsigned)i;
 }
 
-// One thread per particle, particles visited in cell order (perm) so that
-// the scattered adds of neighbouring threads share cache lines; the 2 x 64
-// contributions are added as int64 fixed point (scale 2^e), so the grid is
-// independent of the order of the adds.
-template <typename W>
-__global__ __launch_bounds__(256) void k_assign(const float* __restrict__ xyz, const float* __restrict__ w,
-                                                const unsigned* __restrict__ perm,
-                                                int64_t np, int N, float kf_ks, float clip_max,
-                                                double scale, unsigned long long* __restrict__ ga,
-                                                unsigned long long* __restrict__ gb) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  const int64_t i = perm ? (int64_t)perm[t] : t;
-  float p[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) p[d] = clip_pos(xyz[(int64_t)d * np + i], clip_max);
-  const W we = (W)(w ? w[i] : 1.0f);
-#pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    int cx, cy, cz;
-    W wx[4], wy[4], wz[4];
-    pcs_weights(pcs_coord(p[0], kf_ks, g == 1, N, &cx), wx);
-    pcs_weights(pcs_coord(p[1], kf_ks, g == 1, N, &cy), wy);
-    pcs_weights(pcs_coord(p[2], kf_ks, g == 1, N, &cz), wz);
-    unsigned long long* grid = g == 0 ? ga : gb;
-    int ix[4], iy[4], iz[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      ix[a] = (cx + a - 1 + N) % N;
-      iy[a] = (cy + a - 1 + N) % N;
-      iz[a] = (cz + a - 1 + N) % N;
+// Tiles (width tw) along one axis holding the base cells c in [t0-3, t1]
+// (cyclic) whose stencils reach the cells [t0, t1): grid A covers c-1..c+2
+// and grid B (base c or c+1) c-1..c+3.  At most t1-t0+4 <= 8 tiles.
+__device__ int source_tiles(int t0, int t1, int N, int tw, int* out) {
+  int n = 0;
+  const int span = min(t1 - t0 + 4, N);
+  for (int k = 0; k < span; ++k) {
+    const int c = ((t0 - 3 + k) % N + N) % N;
+    const int t = c / tw;
+    bool seen = false;
+    for (int u = 0; u < n; ++u) seen = seen || out[u] == t;
+    if (!seen) out[n++] = t;
+  }
+  return n;
+}
+
+__device__ int64_t lower_bound_u32(const unsigned* __restrict__ a, int64_t n, unsigned key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int wrap1(int c, int N) {
+  // c in [-1, 2N): one correction
+  return c < 0 ? c + N : (c >= N ? c - N : c);
+}
+
+// LDS: int64 accumulators acc[line][x][grid] (16 B per cell); with FFTX the
+// same memory then holds the complex lines (row stride N+1) and the twiddles.
+template <typename T, int NC, bool FFTX>
+__global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
+    const float* __restrict__ xyz, const float* __restrict__ w, const unsigned* __restrict__ perm,
+    const unsigned* __restrict__ keys, int64_t np, FftShape sh, AssignTiling at, float kf_ks,
+    float clip_max, double scale, const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem_raw);
+  __shared__ int src[64];
+  __shared__ int64_t sbeg[64];
+  __shared__ int64_t pre[65];
+  __shared__ int nsrc_s;
+  const int N = n_of<NC>(sh);
+  const int y0 = (int)(blockIdx.x % at.nty) * at.ty, z0 = (int)(blockIdx.x / at.nty) * at.tz;
+  const int y1 = min(y0 + at.ty, N), z1 = min(z0 + at.tz, N);
+  const int nl = at.ty * at.tz;
+  const int ncell = nl * N;
+  for (int i = threadIdx.x; i < 2 * ncell; i += FFT_THREADS) acc[i] = 0ull;
+  if (threadIdx.x == 0) {
+    int sy[8], sz[8];
+    const int ny = source_tiles(y0, y1, N, at.ty, sy);
+    const int nz = source_tiles(z0, z1, N, at.tz, sz);
+    int n = 0;
+    for (int b = 0; b < nz; ++b)
+      for (int a = 0; a < ny; ++a) src[n++] = sz[b] * at.nty + sy[a];
+    nsrc_s = n;
+  }
+  __syncthreads();
+  const int nsrc = nsrc_s;
+  if ((int)threadIdx.x < nsrc) {
+    const unsigned key = (unsigned)src[threadIdx.x];
+    sbeg[threadIdx.x] = lower_bound_u32(keys, np, key);
+    pre[threadIdx.x + 1] = lower_bound_u32(keys, np, key + 1u);  // range end for now
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    pre[0] = 0;
+    for (int u = 0; u < nsrc; ++u) {
+      s += pre[u + 1] - sbeg[u];
+      pre[u + 1] = s;
     }
+  }
+  __syncthreads();
+  const int64_t total = pre[nsrc];
+  int s = 0;
+  for (int64_t q = threadIdx.x; q < total; q += FFT_THREADS) {
+    while (q >= pre[s + 1]) ++s;
+    const int64_t i = (int64_t)perm[sbeg[s] + (q - pre[s])];
+    float p[3];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int d = 0; d < 3; ++d) p[d] = clip_pos(xyz[(int64_t)d * np + i], clip_max);
+    const T we = (T)(w ? w[i] : 1.0f);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int64_t row = ((int64_t)iz[c] * N + iy[b]) * N;
+    for (int g = 0; g < 2; ++g) {
+      int cy, cz;
+      const float hy = pcs_coord(p[1], kf_ks, g == 1, N, &cy);
+      const float hz = pcs_coord(p[2], kf_ks, g == 1, N, &cz);
+      int ly[4], lz[4];
+      unsigned my = 0, mz = 0;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          // estimator.f:380-508: hx*hy*hz*we, left to right
-          const W v = wmul(wmul(wmul(wx[a], wy[b]), wz[c]), we);
-          const long long q = __double2ll_rn((double)v * scale);
-          atomicAdd(grid + row + ix[a], (unsigned long long)q);
+      for (int a = 0; a < 4; ++a) {
+        const int yy = wrap1(cy + a - 1, N), zz = wrap1(cz + a - 1, N);
+        ly[a] = yy - y0;
+        lz[a] = zz - z0;
+        if (yy >= y0 && yy < y1) my |= 1u << a;
+        if (zz >= z0 && zz < z1) mz |= 1u << a;
+      }
+      if (my == 0 || mz == 0) continue;
+      int cx;
+      T wx[4], wy[4], wz[4];
+      pcs_weights(pcs_coord(p[0], kf_ks, g == 1, N, &cx), wx);
+      pcs_weights(hy, wy);
+      pcs_weights(hz, wz);
+      int ix[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) ix[a] = wrap1(cx + a - 1, N);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (!((mz >> c) & 1u)) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if (!((my >> b) & 1u)) continue;
+          const int row = (lz[c] * at.ty + ly[b]) * N;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            // estimator.f:380-508: hx*hy*hz*we, left to right
+            const T v = wmul(wmul(wmul(wx[a], wy[b]), wz[c]), we);
+            const long long qv = __double2ll_rn((double)v * scale);
+            atomicAdd(acc + 2 * (row + ix[a]) + g, (unsigned long long)qv);
+          }
         }
       }
     }
   }
-}
-
-__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0ull;
+  __syncthreads();
+  const double inv = 1.0 / scale;
+  if constexpr (!FFTX) {
+    for (int e = threadIdx.x; e < ncell; e += FFT_THREADS) {
+      const int line = e / N, x = e - line * N;
+      const int y = y0 + line % at.ty, z = z0 + line / at.ty;
+      if (y < y1 && z < z1)
+        out[((int64_t)z * N + y) * N + x] = mk((T)((double)(long long)acc[2 * e] * inv),
+                                               (T)((double)(long long)acc[2 * e + 1] * inv));
+    }
+  } else {
+    // Convert in place to complex lines of stride N+1.  Cell e moves from
+    // byte 16e to byte sizeof(cplx<T>)*(e + line): for 8-byte complex the
+    // destination lies in cells already read when rounds go up, for 16-byte
+    // complex in cells already read when rounds go down.
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    const int lstride = N + 1;
+    const int nround = (ncell + FFT_THREADS - 1) / FFT_THREADS;
+    for (int r = 0; r < nround; ++r) {
+      const int rr = sizeof(cplx<T>) < 16 ? r : nround - 1 - r;
+      const int e = rr * FFT_THREADS + threadIdx.x;
+      cplx<T> v = mk(T(0), T(0));
+      if (e < ncell)
+        v = mk((T)((double)(long long)acc[2 * e] * inv), (T)((double)(long long)acc[2 * e + 1] * inv));
+      __syncthreads();
+      if (e < ncell) {
+        const int line = e / N;
+        lds[e + line] = v;
+      }
+      __syncthreads();
+    }
+    cplx<T>* twl = lds + nl * lstride;
+    stage_twiddles(twl, tw, N);
+    __syncthreads();
+    fft_tile<NC, 1, T, Tile<T, NC>::MAXLINES, TileCfg<T>::ELEMS>(lds, sh, nl, lstride, twl);
+    for (int e = threadIdx.x; e < ncell; e += FFT_THREADS) {
+      const int line = e / N, x = e - line * N;
+      const int y = y0 + line % at.ty, z = z0 + line / at.ty;
+      if (y < y1 && z < z1) out[((int64_t)z * N + y) * N + x] = lds[line * lstride + x];
+    }
+  }
 }
 
 // Fixed-point exponent: every cell is bounded by 64 * sum|w| (per-axis PCS
@@ -136,66 +275,80 @@ int fixed_point_exponent(double wsum_abs) {
   return e;
 }
 
-int launch_assign(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N, float kf_ks,
-                  float clip_max, double scale, int64_t* ga, int64_t* gb, int prec) {
-  const int64_t G = (int64_t)N * N * N;
-  hipLaunchKernelGGL(k_fill_u64, dim3(2048), dim3(256), 0, ctx->stream, (unsigned long long*)ga, G);
-  PSX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_fill_u64, dim3(2048), dim3(256), 0, ctx->stream, (unsigned long long*)gb, G);
-  PSX_CHECK_LAUNCH();
-  if (np <= 0) return PSX_OK;
-  PSX_REQUIRE(np < (int64_t)1 << 32, "more than 2^32 - 1 particles");
-  // order the particles by base cell (rocPRIM radix sort of 32-bit keys)
-  unsigned *kin, *kout, *vin, *perm;
-  PSX_TRY(ctx_buffer(ctx, "sort_keys_in", (size_t)np * 4, (void**)&kin));
-  PSX_TRY(ctx_buffer(ctx, "sort_keys_out", (size_t)np * 4, (void**)&kout));
-  PSX_TRY(ctx_buffer(ctx, "sort_idx_in", (size_t)np * 4, (void**)&vin));
-  PSX_TRY(ctx_buffer(ctx, "sort_idx_out", (size_t)np * 4, (void**)&perm));
-  hipLaunchKernelGGL(k_cell_keys, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz, np,
-                     N, kf_ks, clip_max, kin, vin);
-  PSX_CHECK_LAUNCH();
-  int bits = 1;
-  while (((int64_t)1 << bits) < G) ++bits;
-  size_t tmp_bytes = 0;
-  PSX_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, perm, (size_t)np, 0, bits,
-                                    ctx->stream));
-  void* tmp;
-  PSX_TRY(ctx_buffer(ctx, "sort_tmp", tmp_bytes > 0 ? tmp_bytes : 16, &tmp));
-  PSX_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, perm, (size_t)np, 0, bits,
-                                    ctx->stream));
-  if (prec == PSX_PREC_F64)
-    hipLaunchKernelGGL(k_assign<double>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
-                       xyz, w, perm, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
-                       (unsigned long long*)gb);
-  else
-    hipLaunchKernelGGL(k_assign<float>, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
-                       xyz, w, perm, np, N, kf_ks, clip_max, scale, (unsigned long long*)ga,
-                       (unsigned long long*)gb);
+template <typename T, int NC>
+static int assign_lines_n(psx_ctx* ctx, const float* xyz, const float* w, const unsigned* perm,
+                          const unsigned* keys, int64_t np, int N, AssignTiling at, float kf_ks,
+                          float clip_max, double scale, cplx<T>* grid, bool fft_x) {
+  const FftPlan* p;
+  PSX_TRY(get_fft_plan(ctx, N, &p));
+  const size_t nl = (size_t)at.ty * at.tz;
+  const size_t acc_bytes = 16 * nl * N;
+  const unsigned nblocks = (unsigned)(at.nty * at.ntz);
+  if (fft_x) {
+    const size_t fft_bytes = sizeof(cplx<T>) * (nl * (N + 1) + N);
+    const size_t lds = acc_bytes > fft_bytes ? acc_bytes : fft_bytes;
+    auto kern = k_assign_lines<T, NC, true>;
+    PSX_TRY(set_lds(kern, lds));
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, xyz, w, perm, keys,
+                       np, p->shape, at, kf_ks, clip_max, scale, plan_tw<T>(p), grid);
+  } else {
+    auto kern = k_assign_lines<T, 0, false>;
+    PSX_TRY(set_lds(kern, acc_bytes));
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, xyz, w, perm,
+                       keys, np, p->shape, at, kf_ks, clip_max, scale, plan_tw<T>(p), grid);
+  }
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
 
-// Fixed-point grids -> interleaved complex grid (re = A, im = B).
+#define PSX_CASE_N(n)                                                                         \
+  case n:                                                                                     \
+    return assign_lines_n<T, n>(ctx, xyz, w, perm, keys, np, N, at, kf_ks, clip_max, scale, \
+                                grid, fft_x);
+
 template <typename T>
-__global__ void k_fixed_to_complex(const int64_t* __restrict__ ga, const int64_t* __restrict__ gb,
-                                   double inv_scale, cplx<T>* __restrict__ out, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = mk((T)((double)ga[i] * inv_scale), (T)((double)gb[i] * inv_scale));
+static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
+                         float kf_ks, float clip_max, double scale, cplx<T>* grid, bool fft_x) {
+  PSX_REQUIRE(np < (int64_t)1 << 32, "more than 2^32 - 1 particles");
+  const AssignTiling at = assign_tiling<T>(N);
+  const unsigned* perm = nullptr;
+  const unsigned* keys = nullptr;
+  if (np > 0) {
+    // order the particles by assignment tile (rocPRIM radix sort of 32-bit keys)
+    unsigned *kin, *kout, *vin, *pout;
+    PSX_TRY(ctx_buffer(ctx, "sort_keys_in", (size_t)np * 4, (void**)&kin));
+    PSX_TRY(ctx_buffer(ctx, "sort_keys_out", (size_t)np * 4, (void**)&kout));
+    PSX_TRY(ctx_buffer(ctx, "sort_idx_in", (size_t)np * 4, (void**)&vin));
+    PSX_TRY(ctx_buffer(ctx, "sort_idx_out", (size_t)np * 4, (void**)&pout));
+    hipLaunchKernelGGL(k_tile_keys, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz,
+                       np, N, kf_ks, clip_max, at, kin, vin);
+    PSX_CHECK_LAUNCH();
+    const int64_t ntiles = (int64_t)at.nty * at.ntz;
+    int bits = 1;
+    while (((int64_t)1 << bits) < ntiles) ++bits;
+    size_t tmp_bytes = 0;
+    PSX_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, pout, (size_t)np, 0, bits,
+                                      ctx->stream));
+    void* tmp;
+    PSX_TRY(ctx_buffer(ctx, "sort_tmp", tmp_bytes > 0 ? tmp_bytes : 16, &tmp));
+    PSX_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, pout, (size_t)np, 0, bits,
+                                      ctx->stream));
+    perm = pout;
+    keys = kout;
+  }
+  if (!fft_x) return assign_lines_n<T, 0>(ctx, xyz, w, perm, keys, np, N, at, kf_ks, clip_max, scale, grid, false);
+  switch (N) {
+    PSX_FFT_SIZES(PSX_CASE_N)
+    default: return assign_lines_n<T, 0>(ctx, xyz, w, perm, keys, np, N, at, kf_ks, clip_max, scale, grid, true);
+  }
 }
+#undef PSX_CASE_N
 
-int launch_fixed_to_complex(psx_ctx* ctx, const int64_t* ga, const int64_t* gb, double inv_scale,
-                            void* grid, int N, int prec) {
-  const int64_t n = (int64_t)N * N * N;
-  const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), 8192);
+int launch_assign_grid(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
+                       float kf_ks, float clip_max, double scale, void* grid, int prec, bool fft_x) {
   if (prec == PSX_PREC_F64)
-    hipLaunchKernelGGL(k_fixed_to_complex<double>, dim3(blocks), dim3(256), 0, ctx->stream, ga, gb,
-                       inv_scale, (cplx<double>*)grid, n);
-  else
-    hipLaunchKernelGGL(k_fixed_to_complex<float>, dim3(blocks), dim3(256), 0, ctx->stream, ga, gb,
-                       inv_scale, (cplx<float>*)grid, n);
-  PSX_CHECK_LAUNCH();
-  return PSX_OK;
+    return assign_grid_t<double>(ctx, xyz, w, np, N, kf_ks, clip_max, scale, (cplx<double>*)grid, fft_x);
+  return assign_grid_t<float>(ctx, xyz, w, np, N, kf_ks, clip_max, scale, (cplx<float>*)grid, fft_x);
 }
 
 // Unit field delta = 1 (the shell indicators of _counts_Bk123), complex double.
