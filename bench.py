@@ -37,12 +37,18 @@ def parse():
     ap.add_argument("--npart", type=int, default=1_000_000)
     ap.add_argument("--ngrid", type=int, default=360)
     ap.add_argument("--nmax", type=int, default=40)
+    ap.add_argument("--step", type=int, default=3)
+    ap.add_argument("--ncut", type=int, default=3)
+    ap.add_argument("--lbox", type=float, default=2500.0)
+    ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-triangles", type=int, default=12)
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--config", default="box", choices=["box", "serve", "batch"],
+    ap.add_argument("--config", default="box", choices=["box", "serve", "batch", "pmc"],
                     help="box: the single clustered box (default); serve: Poisson request "
-                         "stream at --rate/s; batch: 64 uneven boxes submitted at once")
+                         "stream at --rate/s; batch: 64 uneven boxes submitted at once; "
+                         "pmc: one box, a 1 GiB device copy (counter calibration), one box "
+                         "-- the program to run under rocprofv3 --pmc")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--nreq", type=int, default=40)
     ap.add_argument("--slots", type=int, default=4)
@@ -135,6 +141,37 @@ def stage_times(box, prec, reps=3):
     return out
 
 
+def accuracy(out, p0k, out64, p0k64, cnt):
+    """Errors of a run against the FP64 GPU run of the same box: B before shot
+    noise (B_raw = b123 + b123_sn), B with the denominator max(|B_fp64|,
+    b123_sn_fp64) (B ~ 0 for shot-noise-dominated triangles), Q and P(k_j)."""
+    m = cnt > 0
+    braw, braw64 = out[3][m] + out[6][m], out64[3][m] + out64[6][m]
+    den = np.maximum(np.abs(out64[3][m]), np.abs(out64[6][m]))
+    pm = np.isfinite(p0k64) & (p0k64 != 0)
+    return {"b_raw_rel_max": float(np.max(np.abs(braw - braw64) / np.abs(braw64))),
+            "b_rel_max_sn_floor": float(np.max(np.abs(out[3][m] - out64[3][m]) / den)),
+            "q_rel_max_sn_floor": float(np.max(np.abs(out[4][m] - out64[4][m]) /
+                                               np.maximum(np.abs(out64[4][m]), 1.0))),
+            "p0k_rel_max": float(np.max(np.abs(p0k[pm] - p0k64[pm]) / np.abs(p0k64[pm]))),
+            "triangles": int(m.sum())}
+
+
+def contraction_traffic(N, Nmax, prec):
+    """HBM bytes per contraction launch measured with rocprofv3 --pmc
+    (FETCH_SIZE x 2 + WRITE_SIZE, the factor 2 from a 1 GiB copy kernel), kept
+    as a committed summary next to the command that produced it; None when no
+    measurement exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    v = tab.get("k_contract/%s/N%d/Nmax%d" % (prec, N, Nmax))
+    return None if v is None else float(v)
+
+
 def cpu_baseline(xyz_np, Lbox, Ngrid, step, Ncut, Nmax, n_tri_sample):
     """The oracle (FP64 numpy restatement of the reference path, 1 thread) timed
     on a bounded sample of the same box: the full front end (assignment, FFT,
@@ -200,8 +237,29 @@ def run_stream(args):
         "mean_ms": round(st["mean_ms"], 2), "wall_s": round(wall, 3)}))
 
 
+def run_pmc(args):
+    """Short program for hardware-counter passes: the box once (plans, counts
+    and buffers warm), a 1 GiB device-to-device copy whose FETCH_SIZE and
+    WRITE_SIZE calibrate the counters, then the box once more."""
+    import torch
+    from pyspectrum_amd import mocks
+    xyz, _ = mocks.lognormal_box(args.npart, args.lbox, seed=args.seed)
+    box = Box(xyz, args.lbox, args.ngrid, args.step, args.ncut, args.nmax)
+    box.run(args.precision)
+    torch.cuda.synchronize()
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = a.clone()
+    torch.cuda.synchronize()
+    del a, b
+    box.run(args.precision)
+    torch.cuda.synchronize()
+    print("pmc program done: %s, Ngrid=%d, Nmax=%d" % (args.precision, args.ngrid, args.nmax))
+
+
 def main():
     args = parse()
+    if args.gpus == 1 and args.config == "pmc":
+        return run_pmc(args)
     if args.gpus == 1 and args.config != "box":
         return run_stream(args)
     if args.gpus != 1:
@@ -213,9 +271,9 @@ def main():
     from pyspectrum_amd import _lib as L
     from pyspectrum_amd import mocks
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    Lbox, N, step, Ncut, Nmax = 2500.0, args.ngrid, 3, 3, args.nmax
+    Lbox, N, step, Ncut, Nmax = args.lbox, args.ngrid, args.step, args.ncut, args.nmax
     t0 = time.perf_counter()
-    xyz, A = mocks.lognormal_box(args.npart, Lbox, seed=2)
+    xyz, A = mocks.lognormal_box(args.npart, Lbox, seed=args.seed)
     torch.cuda.synchronize()
     if args.verbose:
         print("mock generated in %.1fs (A=%.4g)" % (time.perf_counter() - t0, A), file=sys.stderr, flush=True)
@@ -225,8 +283,11 @@ def main():
 
     wall, lat = time_steps(box, args.precision, args.steps, args.warmup)
     value = args.steps / wall
+    out_mixed, p0k_mixed = box.out7.cpu().numpy().copy(), box.p0k.cpu().numpy().copy()
     wall64, lat64 = time_steps(box, "fp64", args.fp64_steps, 1)
     v64 = args.fp64_steps / wall64
+    err = accuracy(out_mixed, p0k_mixed, box.out7.cpu().numpy(), box.p0k.cpu().numpy(),
+                   box.cnt.cpu().numpy())
     st = stage_times(box, args.precision)
     st64 = stage_times(box, "fp64")
     if args.verbose:
@@ -243,7 +304,7 @@ def main():
         PEAK["fp32_tflops"] if args.precision == "fp32" else PEAK["fp64_tflops"])
     roofline = {"bound": "mfma", "kernel": "k_contract", "achieved": round(achieved, 2),
                 "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                "traffic": None}
+                "traffic": contraction_traffic(N, Nmax, args.precision)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -255,15 +316,17 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"bf16x3": "bf16x3+f32", "fp32": "f32", "fp64": "f64"}[args.precision],
-        "data": "synthetic lognormal mock (seed 2), generated on the GPU",
-        "config": {"workload": "single clustered box: 1e6 lognormal points, Lbox=2500, Ngrid=%d, "
-                               "step=3, Ncut=3, Nmax=%d (%d triangles)" % (N, Nmax, box.ntri),
+        "data": "synthetic lognormal mock (seed %d), generated on the GPU" % args.seed,
+        "config": {"workload": "single clustered box: %.3g lognormal points, Lbox=%g, Ngrid=%d, "
+                               "step=%d, Ncut=%d, Nmax=%d (%d triangles)"
+                               % (args.npart, Lbox, N, step, Ncut, Nmax, box.ntri),
                    "precision": args.precision},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "fp64_boxes_per_s": round(v64, 3),
         "fp64_p50_ms": round(float(np.percentile(lat64, 50)), 3),
         "mixed_vs_fp64": round(value / v64, 3),
+        "error_vs_fp64": err,
         "stages_ms": {k: round(v, 3) for k, v in st.items()},
         "stages_fp64_ms": {k: round(v, 3) for k, v in st64.items()},
         "roofline": roofline,

@@ -36,13 +36,12 @@ namespace psx {
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
-typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 
-// Compile-time tuning: waves per workgroup, pair tiles per wave in the
-// split-BF16 mode, and the minimum waves per SIMD that register allocation
-// must allow (1 = no constraint).
+// Compile-time tuning: waves per workgroup, pair tiles per wave and points
+// per chunk in the split-BF16 mode, and the minimum waves per SIMD that
+// register allocation must allow (1 = no constraint).
 #ifndef PSX_CT_WAVES
 #define PSX_CT_WAVES 8
 #endif
@@ -51,6 +50,9 @@ typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 #endif
 #ifndef PSX_CT_MINW
 #define PSX_CT_MINW 1
+#endif
+#ifndef PSX_CT_KC_BF
+#define PSX_CT_KC_BF 128
 #endif
 constexpr int CT_WAVES = PSX_CT_WAVES;
 constexpr int CT_THREADS = CT_WAVES * 64;
@@ -62,13 +64,19 @@ template <> struct CtTraits<PSX_PREC_F64> {
   static constexpr int KC = 128, MAXPT = 4, PAD = 2;
 };
 template <> struct CtTraits<PSX_PREC_F32> {
-  typedef float TD; typedef f4v Acc; typedef float TP;
-  static constexpr int KC = 128, MAXPT = 4, PAD = 4;
+  typedef float TD; typedef f4v Acc; typedef double TP;
+  static constexpr int KC = 128, MAXPT = 2, PAD = 4;
 };
 template <> struct CtTraits<PSX_PREC_BF16X3> {
-  typedef float TD; typedef f4v Acc; typedef float TP;
-  static constexpr int KC = 128, MAXPT = PSX_CT_MAXPT_BF, PAD = 4;
+  typedef float TD; typedef f4v Acc; typedef double TP;
+  static constexpr int KC = PSX_CT_KC_BF, MAXPT = PSX_CT_MAXPT_BF, PAD = 4;
 };
+
+// Pair tiles per wave: the mode's MAXPT, one fewer in the split-BF16 mode at
+// 4 l tiles (its float32 and float64 accumulators must fit in 256 VGPRs).
+template <int MODE, int NLT> __host__ __device__ constexpr int ct_maxpt() {
+  return (MODE == PSX_PREC_BF16X3 && NLT > 3) ? CtTraits<MODE>::MAXPT - 1 : CtTraits<MODE>::MAXPT;
+}
 
 // Row strides (elements): padded so that 8 consecutive rows start in
 // different 16-byte bank groups and every row start stays 16-byte aligned.
@@ -86,14 +94,41 @@ static size_t ct_lds_bytes(int ns) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// x = hi + lo with hi = bf16(x) (round to nearest even), lo = bf16(x - hi)
-__device__ __forceinline__ void split_bf16(const float (&x)[8], bf8v& hi, bf8v& lo) {
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+// Two floats -> packed bf16 pair (round to nearest even; v_cvt_pk_bf16_f32).
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  f2v x;
+  x[0] = a; x[1] = b;
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(x, bf2v));
+}
+
+// Products p = a*b of 8 point pairs, split as p = hi + lo: hi = bf16(p), lo =
+// bf16(a*b - hi) with the residual formed exactly by one FMA.  Per value:
+// a multiply, half a pack, a shift or mask back to float, an FMA, half a pack.
+__device__ __forceinline__ void product_split_bf16(const f4v& a0, const f4v& a1, const f4v& b0,
+                                                   const f4v& b1, bf8v& hi, bf8v& lo) {
+  float a[8], b[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const __bf16 h = (__bf16)x[k];
-    hi[k] = h;
-    lo[k] = (__bf16)(x[k] - (float)h);
+  for (int e = 0; e < 4; ++e) {
+    a[e] = a0[e]; a[4 + e] = a1[e];
+    b[e] = b0[e]; b[4 + e] = b1[e];
   }
+  u4v h, l;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const float x0 = a[2 * p] * b[2 * p], x1 = a[2 * p + 1] * b[2 * p + 1];
+    const unsigned hb = pack_bf16(x0, x1);
+    const float h0 = __builtin_bit_cast(float, hb << 16);
+    const float h1 = __builtin_bit_cast(float, hb & 0xffff0000u);
+    h[p] = hb;
+    l[p] = pack_bf16(fmaf(a[2 * p], b[2 * p], -h0), fmaf(a[2 * p + 1], b[2 * p + 1], -h1));
+  }
+  hi = __builtin_bit_cast(bf8v, h);
+  lo = __builtin_bit_cast(bf8v, l);
 }
 
 // D: [ns][G] shell fields; partials: [gridDim.y][gridDim.x][nslot][256].
@@ -107,7 +142,7 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
   typedef CtTraits<MODE> Tr;
   typedef typename Tr::TD TD;
   typedef typename Tr::Acc Acc;
-  constexpr int KC = Tr::KC, MAXPT = Tr::MAXPT;
+  constexpr int KC = Tr::KC, MAXPT = ct_maxpt<MODE, NLT>();
   constexpr int LDW = ct_ldw<MODE>();
   constexpr int LDWH = ct_ldwh<MODE>();
   constexpr bool BF = MODE == PSX_PREC_BF16X3;
@@ -151,13 +186,26 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
     ob[t] = ((s < ns) ? s : ns) * LDW + kbase;
     obh[t] = ((s < ns) ? s : ns) * LDWH + kbase;
   }
+  // F32/BF16X3: the MFMA accumulators restart from zero every chunk and are
+  // added into float64 accumulators (acc2) at the end of the chunk, so no
+  // float32 accumulation chain is longer than one chunk.  (The bf16 MFMA's
+  // internal sum of 32 products rounds low; over a long chain that bias is
+  // systematic.)
+  constexpr bool TWO = MODE != PSX_PREC_F64;
+  constexpr int M2 = TWO ? MAXPT : 1, N2 = TWO ? NLT : 1;
   Acc acc[MAXPT][NLT];
+  d4v acc2[M2][N2];
+  const Acc zero = {0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < MAXPT; ++q)
 #pragma unroll
-    for (int t = 0; t < NLT; ++t)
+    for (int t = 0; t < NLT; ++t) acc[q][t] = zero;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc[q][t][e] = 0;
+  for (int q = 0; q < M2; ++q)
+#pragma unroll
+    for (int t = 0; t < N2; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc2[q][t][e] = 0.0;
 
   const int64_t x_begin = (int64_t)blockIdx.x * pts_per_block;
   const int64_t x_end = min(G, x_begin + pts_per_block);
@@ -208,14 +256,18 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
           *reinterpret_cast<d2v*>(lds + s * LDW + k + 2) = v1;
         }
         if constexpr (BF) {
-          bf4v h, l;
+          // x = hi + lo, hi = bf16(x), lo = bf16(x - hi) (x - hi is exact)
+          u2v h, l;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            h[e] = (__bf16)(float)pre[u][e];
-            l[e] = (__bf16)((float)pre[u][e] - (float)h[e]);
+          for (int p = 0; p < 2; ++p) {
+            const float x0 = (float)pre[u][2 * p], x1 = (float)pre[u][2 * p + 1];
+            const unsigned hb = pack_bf16(x0, x1);
+            h[p] = hb;
+            l[p] = pack_bf16(x0 - __builtin_bit_cast(float, hb << 16),
+                             x1 - __builtin_bit_cast(float, hb & 0xffff0000u));
           }
-          *reinterpret_cast<bf4v*>(ldh + s * LDWH + k) = h;
-          *reinterpret_cast<bf4v*>(ldl + s * LDWH + k) = l;
+          *reinterpret_cast<u2v*>(ldh + s * LDWH + k) = h;
+          *reinterpret_cast<u2v*>(ldl + s * LDWH + k) = l;
         }
       }
     }
@@ -239,19 +291,12 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
           if (q >= nq) break;
           const f4v* pa = reinterpret_cast<const f4v*>(lds + oi[q] + k0);
           const f4v* pb = reinterpret_cast<const f4v*>(lds + oj[q] + k0);
-          const f4v a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
-          float xa[8];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            xa[e] = a0[e] * b0[e];
-            xa[4 + e] = a1[e] * b1[e];
-          }
           bf8v ah, al;
-          split_bf16(xa, ah, al);
+          product_split_bf16(pa[0], pa[1], pb[0], pb[1], ah, al);
 #pragma unroll
           for (int t = 0; t < NLT; ++t) {
             if ((msk[q] >> t) & 1) {
-              acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[q][t], 0, 0, 0);
+              acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], k0 == 0 ? zero : acc[q][t], 0, 0, 0);
               acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[q][t], 0, 0, 0);
               acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[q][t], 0, 0, 0);
             }
@@ -297,11 +342,24 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
                 if constexpr (MODE == PSX_PREC_F64)
                   acc[q][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[e], b[t][e], acc[q][t], 0, 0, 0);
                 else
-                  acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[t][e], acc[q][t], 0, 0, 0);
+                  acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                      a[e], b[t][e], (s0 == 0 && e == 0) ? zero : acc[q][t], 0, 0, 0);
               }
             }
           }
         }
+      }
+    }
+    if constexpr (TWO) {
+#pragma unroll
+      for (int q = 0; q < M2; ++q) {
+        if (q >= nq) break;
+#pragma unroll
+        for (int t = 0; t < N2; ++t)
+          if ((msk[q] >> t) & 1) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc2[q][t][e] += (double)acc[q][t][e];
+          }
       }
     }
     __syncthreads();
@@ -315,7 +373,10 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
       const int slot = uniform(wave_slot[(gw * MAXPT + q) * CT_MAXLT + t]);
       if (slot < 0) continue;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) out[(int64_t)slot * 256 + lane * 4 + e] = acc[q][t][e];
+      for (int e = 0; e < 4; ++e) {
+        if constexpr (TWO) out[(int64_t)slot * 256 + lane * 4 + e] = acc2[q][t][e];
+        else out[(int64_t)slot * 256 + lane * 4 + e] = acc[q][t][e];
+      }
     }
   }
 }
@@ -378,6 +439,7 @@ static int contract_slots(psx_ctx* ctx, size_t lds, int* out) {
 template <int MODE, int NLT>
 static int contract_t(psx_ctx* ctx, const ContractPlan* cp, const void* D, int N, double* braw) {
   typedef CtTraits<MODE> Tr;
+  PSX_REQUIRE(cp->maxpt == (ct_maxpt<MODE, NLT>()), "contraction plan built for another precision");
   const int64_t G = (int64_t)N * N * N;
   const int ngroups = cp->waves / CT_WAVES;
   const size_t lds = ct_lds_bytes<MODE>(cp->ns);
@@ -416,7 +478,6 @@ static int contract_mode(psx_ctx* ctx, const void* D, int N, int step, int ncut,
                          double* braw) {
   const ContractPlan* cp;
   PSX_TRY(get_contract_plan(ctx, MODE, step, ncut, nmax, &cp));
-  PSX_REQUIRE(cp->maxpt == CtTraits<MODE>::MAXPT, "contraction plan built for another precision");
   switch (cp->nlt) {
     case 1: return contract_t<MODE, 1>(ctx, cp, D, N, braw);
     case 2: return contract_t<MODE, 2>(ctx, cp, D, N, braw);
@@ -428,10 +489,10 @@ static int contract_mode(psx_ctx* ctx, const void* D, int N, int step, int ncut,
 
 int contract_waves_per_group() { return CT_WAVES; }
 
-int contract_maxpt(int prec) {
+int contract_maxpt(int prec, int nlt) {
   if (prec == PSX_PREC_F64) return CtTraits<PSX_PREC_F64>::MAXPT;
   if (prec == PSX_PREC_F32) return CtTraits<PSX_PREC_F32>::MAXPT;
-  return CtTraits<PSX_PREC_BF16X3>::MAXPT;
+  return nlt > 3 ? ct_maxpt<PSX_PREC_BF16X3, 4>() : ct_maxpt<PSX_PREC_BF16X3, 3>();
 }
 
 int launch_contract(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax, int prec,

@@ -71,9 +71,17 @@ struct AssignTiling {
   int ty, tz, nty, ntz;  // lines per tile along y and z; tiles along y and z
 };
 
+// At most 8 lines per tile keeps the LDS of a tile (16 B per cell plus the
+// complex lines) small enough for several workgroups per CU.
+constexpr int ASSIGN_LINES = 8;
+template <typename T, int NC>
+constexpr int assign_lines_max() {
+  return Tile<T, NC>::MAXLINES < ASSIGN_LINES ? Tile<T, NC>::MAXLINES : ASSIGN_LINES;
+}
+
 template <typename T>
 static AssignTiling assign_tiling(int N) {
-  const int tl = tile_lines_host<T>(N);
+  const int tl = std::min(tile_lines_host<T>(N), ASSIGN_LINES);
   AssignTiling a;
   a.ty = tl >= 4 ? 4 : tl;
   a.tz = tl / a.ty;
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
     cplx<T>* twl = lds + nl * lstride;
     stage_twiddles(twl, tw, N);
     __syncthreads();
-    fft_tile<NC, 1, T, Tile<T, NC>::MAXLINES, TileCfg<T>::ELEMS>(lds, sh, nl, lstride, twl);
+    fft_tile<NC, 1, T, assign_lines_max<T, NC>(), TileCfg<T>::ELEMS>(lds, sh, nl, lstride, twl);
     for (int e = threadIdx.x; e < ncell; e += FFT_THREADS) {
       const int line = e / N, x = e - line * N;
       const int y = y0 + line % at.ty, z = z0 + line / at.ty;

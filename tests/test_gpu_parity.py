@@ -317,3 +317,39 @@ def test_plain_config_matches_oracle():
     pk = P.Pk_periodic(xyz, Lbox=Lbox, Ngrid=N, precision="fp64")
     assert np.array_equal(pk["counts"], pk_ref["counts"])
     assert _rel(pk["p0k"], pk_ref["p0k"]) < 1e-10
+
+
+# ------------------------------------------------- precision-stress config -----
+@pytest.mark.parametrize("nmax,ntri", [(60, 20275), (30, 2825)])
+def test_precision_stress_config(nmax, ntri):
+    """4e6 lognormal points (seed 4) in a 3000 Mpc/h box, Ngrid=512, step=2,
+    Ncut=2: Nmax=60 (60 shells, the literal API reading) and Nmax=30.  Too
+    large for the oracle, so checked through properties: the GPU's FP64
+    counts are integer multiples of G, and the FP32 and split-BF16 paths agree
+    with the FP64 path on B with the shot-noise floor max(|B|, B_sn) (1e-5
+    FP32, 1e-3 split-BF16), on B before shot noise (1e-4 / 1e-3: at Ngrid=512
+    the FP32 shell FFTs, which the reference also runs in FP32, set a ~2e-5
+    floor on the most cancelling raw sums) and on P(k_j)."""
+    from pyspectrum_amd import mocks
+    _say("precision stress Ngrid=512 step=2 Nmax=%d" % nmax)
+    xyz, _ = mocks.lognormal_box(4_000_000, 3000.0, seed=4)
+    xyz = xyz.cpu().numpy()
+    N = 512
+    counts = P._counts_Bk123(Ngrid=N, Nmax=nmax, Ncut=2, step=2)
+    ijl = L.triangles(2, 2, nmax)
+    assert len(ijl) == ntri
+    c = counts[ijl[:, 0] - 1, ijl[:, 1] - 1, ijl[:, 2] - 1] / float(N) ** 3
+    assert np.max(np.abs(c - np.rint(c))) < 1e-3
+    out = {p: P.Bk_periodic(xyz, Lbox=3000.0, Ngrid=N, step=2, Ncut=2, Nmax=nmax, precision=p)
+           for p in ("fp64", "fp32", "bf16x3")}
+    ref = out["fp64"]
+    keep = ref["counts"] > 0
+    braw64 = (ref["b123"] + ref["b123_sn"])[keep]
+    den = np.maximum(np.abs(ref["b123"]), np.abs(ref["b123_sn"]))[keep]
+    for p, tol, tol_raw in (("fp32", 1e-5, 1e-4), ("bf16x3", 1e-3, 1e-3)):
+        got = out[p]
+        assert np.array_equal(got["counts"], ref["counts"]), p
+        braw = (got["b123"] + got["b123_sn"])[keep]
+        assert np.max(np.abs(braw - braw64) / np.abs(braw64)) < tol_raw, p
+        assert np.max(np.abs(got["b123"][keep] - ref["b123"][keep]) / den) < tol, p
+        assert _rel(got["p0k1"], ref["p0k1"]) < tol, p
