@@ -187,7 +187,7 @@ int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, co
     for (int z = 0; z < N; ++z)
       for (int kx0 = 0; kx0 < xlim; kx0 += kt) yt.push_back({s, z, kx0, 0});
     for (int z = 0; z < N; ++z)
-      for (int y0 = 0; y0 < N; y0 += 16) xt.push_back({s, z, y0, 0});
+      for (int y0 = 0; y0 < N; y0 += 2 * SHELL_X_PAIRS) xt.push_back({s, z, y0, 0});
   }
   sp->t_elems = off;
   sp->nz_tiles = (int)zt.size();

@@ -30,6 +30,10 @@ struct FftPlan {
   double* Wd = nullptr;
 };
 
+// Row pairs (complex FFT lines) per tile of the shell fields' x pass: small
+// tiles keep several workgroups per CU to hide the gather latency.
+constexpr int SHELL_X_PAIRS = 8;
+
 // One pruned-pass tile: (shell slot, a, b) with meaning per pass.
 struct Tile3 {
   int s, a, b, pad;

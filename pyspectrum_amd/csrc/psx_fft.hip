@@ -195,16 +195,20 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
   }
 }
 
-// x pass (Hermitian -> real): tile (s, z, y0) covers rows y0..y0+15 as 8 pairs;
-// each pair is one complex FFT of X1 + i X2 whose real/imag parts are the two
-// real rows.  kx = 0 and kx = N/2 keep only their real part, which makes the
-// result exactly Re FFT of the Hermitian extension.
+// x pass (Hermitian -> real): tile (s, z, y0) covers rows y0..y0+2NP-1 as NP
+// pairs (NP = SHELL_X_PAIRS); each pair is one complex FFT of
+// X1 + i X2 whose real/imag parts are the two real rows.  kx = 0 and kx = N/2
+// keep only their real part, which makes the result exactly Re FFT of the
+// Hermitian extension.  Thread (p, lane) works on pair p throughout, so the
+// gather, zero fill and store loops need no index division.
 template <typename T, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restrict__ T2,
                                                          T* __restrict__ D,
                                                          const Tile3* __restrict__ tiles,
                                                          ShellDev sd, FftShape sh,
                                                          const cplx<T>* __restrict__ tw) {
+  constexpr int NP = SHELL_X_PAIRS;
+  constexpr int TPL = FFT_THREADS / NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int N = n_of<NC>(sh), lstride = N + 1, hN = N / 2;
@@ -212,37 +216,36 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
   const int s = t.s, z = t.a, y0 = t.b;
   const int kxw = sd.meta[4 * s + 2];
   const int kxl = min(kxw, hN + 1);
-  const int npair = min(8, (N - y0) / 2);
-  for (int idx = threadIdx.x; idx < npair * N; idx += FFT_THREADS) {
-    const int p = idx / N, n = idx - p * N;
-    lds[p * lstride + n] = mk(T(0), T(0));
-  }
-  cplx<T>* twl = lds + 8 * lstride;
+  const int npair = min(NP, (N - y0) / 2);
+  const int p = (int)threadIdx.x / TPL, lane = (int)threadIdx.x - p * TPL;
+  cplx<T>* L = lds + p * lstride;
+  // zero the modes no gather writes: [kxl, N - kxl]
+  if (p < npair)
+    for (int n = kxl + lane; n <= N - kxl; n += TPL) L[n] = mk(T(0), T(0));
+  cplx<T>* twl = lds + NP * lstride;
   stage_twiddles(twl, tw, N);
-  __syncthreads();
-  const int64_t base = sd.off[s] + (int64_t)z * N * kxw;
-  for (int idx = threadIdx.x; idx < npair * kxl; idx += FFT_THREADS) {
-    const int p = idx / kxl, kx = idx - p * kxl;
-    const int y1 = y0 + 2 * p;
-    cplx<T> a = T2[base + (int64_t)y1 * kxw + kx];
-    cplx<T> b = T2[base + (int64_t)(y1 + 1) * kxw + kx];
-    cplx<T>* L = lds + p * lstride;
-    if (kx == 0 || kx == hN) {
-      L[kx] = mk(a.re, b.re);
-    } else {
-      L[kx] = mk(a.re - b.im, a.im + b.re);
-      L[N - kx] = mk(a.re + b.im, b.re - a.im);
+  if (p < npair) {
+    const cplx<T>* r1 = T2 + sd.off[s] + ((int64_t)z * N + y0 + 2 * p) * kxw;
+    const cplx<T>* r2 = r1 + kxw;
+    for (int kx = lane; kx < kxl; kx += TPL) {
+      const cplx<T> a = r1[kx], b = r2[kx];
+      if (kx == 0 || kx == hN) {
+        L[kx] = mk(a.re, b.re);
+      } else {
+        L[kx] = mk(a.re - b.im, a.im + b.re);
+        L[N - kx] = mk(a.re + b.im, b.re - a.im);
+      }
     }
   }
   __syncthreads();
-  fft_tile<NC, -1, T, 8, 8 * 1024>(lds, sh, npair, lstride, twl);
+  fft_tile<NC, -1, T, NP, NP * 1024>(lds, sh, npair, lstride, twl);
+  // rows y0 .. y0 + 2 npair - 1 are contiguous in D: one coalesced stream
   const int64_t G = (int64_t)N * N * N;
   T* dst = D + (int64_t)s * G + ((int64_t)z * N + y0) * N;
-  for (int idx = threadIdx.x; idx < npair * N; idx += FFT_THREADS) {
-    const int p = idx / N, n = idx - p * N;
-    const cplx<T> v = lds[p * lstride + n];
-    dst[(int64_t)(2 * p) * N + n] = v.re;
-    dst[(int64_t)(2 * p + 1) * N + n] = v.im;
+  for (int idx = threadIdx.x; idx < 2 * npair * N; idx += FFT_THREADS) {
+    const int r = idx / N, n = idx - r * N;
+    const cplx<T> v = lds[(r >> 1) * lstride + n];
+    dst[idx] = (r & 1) ? v.im : v.re;
   }
 }
 
@@ -252,7 +255,7 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
   const int N = p->shape.N;
   ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
   const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
-  const size_t ldsx = ((size_t)8 * (N + 1) + N) * sizeof(cplx<T>);
+  const size_t ldsx = ((size_t)SHELL_X_PAIRS * (N + 1) + N) * sizeof(cplx<T>);
   const cplx<T>* tw = plan_tw<T>(p);
   if (sp->nz_tiles > 0) {
     PSX_TRY(set_lds(k_shell_z<T, NC>, ldsz));
