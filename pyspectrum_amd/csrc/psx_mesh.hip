@@ -103,6 +103,18 @@ __global__ void k_tile_keys(const float* __restrict__ xyz, int64_t np, int N, fl
   idx[i] = (unsigned)i;
 }
 
+// Particles in sorted order as (x, y, z, w) records, positions already
+// clipped, so the tile kernel reads contiguous 16-byte records.
+__global__ void k_gather_particles(const float* __restrict__ xyz, const float* __restrict__ w,
+                                   const unsigned* __restrict__ perm, int64_t np, float clip_max,
+                                   float4* __restrict__ rec) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const int64_t i = (int64_t)perm[t];
+  rec[t] = make_float4(clip_pos(xyz[i], clip_max), clip_pos(xyz[np + i], clip_max),
+                       clip_pos(xyz[2 * np + i], clip_max), w ? w[i] : 1.0f);
+}
+
 // Tiles (width tw) along one axis holding the base cells c in [t0-3, t1]
 // (cyclic) whose stencils reach the cells [t0, t1): grid A covers c-1..c+2
 // and grid B (base c or c+1) c-1..c+3.  At most t1-t0+4 <= 8 tiles.
@@ -138,9 +150,9 @@ __device__ __forceinline__ int wrap1(int c, int N) {
 // same memory then holds the complex lines (row stride N+1) and the twiddles.
 template <typename T, int NC, bool FFTX>
 __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
-    const float* __restrict__ xyz, const float* __restrict__ w, const unsigned* __restrict__ perm,
-    const unsigned* __restrict__ keys, int64_t np, FftShape sh, AssignTiling at, float kf_ks,
-    float clip_max, double scale, const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ out) {
+    const float4* __restrict__ rec, const unsigned* __restrict__ keys, int64_t np, FftShape sh,
+    AssignTiling at, float kf_ks, double scale, const cplx<T>* __restrict__ tw,
+    cplx<T>* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem_raw);
   __shared__ int src[64];
@@ -183,11 +195,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
   int s = 0;
   for (int64_t q = threadIdx.x; q < total; q += FFT_THREADS) {
     while (q >= pre[s + 1]) ++s;
-    const int64_t i = (int64_t)perm[sbeg[s] + (q - pre[s])];
-    float p[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) p[d] = clip_pos(xyz[(int64_t)d * np + i], clip_max);
-    const T we = (T)(w ? w[i] : 1.0f);
+    const float4 r = rec[sbeg[s] + (q - pre[s])];
+    const float p[3] = {r.x, r.y, r.z};
+    const T we = (T)r.w;
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
       int cy, cz;
@@ -284,9 +294,8 @@ int fixed_point_exponent(double wsum_abs) {
 }
 
 template <typename T, int NC>
-static int assign_lines_n(psx_ctx* ctx, const float* xyz, const float* w, const unsigned* perm,
-                          const unsigned* keys, int64_t np, int N, AssignTiling at, float kf_ks,
-                          float clip_max, double scale, cplx<T>* grid, bool fft_x) {
+static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* keys, int64_t np, int N,
+                          AssignTiling at, float kf_ks, double scale, cplx<T>* grid, bool fft_x) {
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
   const size_t nl = (size_t)at.ty * at.tz;
@@ -297,13 +306,13 @@ static int assign_lines_n(psx_ctx* ctx, const float* xyz, const float* w, const 
     const size_t lds = acc_bytes > fft_bytes ? acc_bytes : fft_bytes;
     auto kern = k_assign_lines<T, NC, true>;
     PSX_TRY(set_lds(kern, lds));
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, xyz, w, perm, keys,
-                       np, p->shape, at, kf_ks, clip_max, scale, plan_tw<T>(p), grid);
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, rec, keys, np,
+                       p->shape, at, kf_ks, scale, plan_tw<T>(p), grid);
   } else {
     auto kern = k_assign_lines<T, 0, false>;
     PSX_TRY(set_lds(kern, acc_bytes));
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, xyz, w, perm,
-                       keys, np, p->shape, at, kf_ks, clip_max, scale, plan_tw<T>(p), grid);
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, keys,
+                       np, p->shape, at, kf_ks, scale, plan_tw<T>(p), grid);
   }
   PSX_CHECK_LAUNCH();
   return PSX_OK;
@@ -311,23 +320,25 @@ static int assign_lines_n(psx_ctx* ctx, const float* xyz, const float* w, const 
 
 #define PSX_CASE_N(n)                                                                         \
   case n:                                                                                     \
-    return assign_lines_n<T, n>(ctx, xyz, w, perm, keys, np, N, at, kf_ks, clip_max, scale, \
-                                grid, fft_x);
+    return assign_lines_n<T, n>(ctx, rec, keys, np, N, at, kf_ks, scale, grid, fft_x);
 
 template <typename T>
 static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
                          float kf_ks, float clip_max, double scale, cplx<T>* grid, bool fft_x) {
   PSX_REQUIRE(np < (int64_t)1 << 32, "more than 2^32 - 1 particles");
   const AssignTiling at = assign_tiling<T>(N);
-  const unsigned* perm = nullptr;
+  const float4* rec = nullptr;
   const unsigned* keys = nullptr;
   if (np > 0) {
-    // order the particles by assignment tile (rocPRIM radix sort of 32-bit keys)
+    // order the particles by assignment tile (rocPRIM radix sort of 32-bit
+    // keys), then gather them into sorted (x, y, z, w) records
     unsigned *kin, *kout, *vin, *pout;
+    float4* r;
     PSX_TRY(ctx_buffer(ctx, "sort_keys_in", (size_t)np * 4, (void**)&kin));
     PSX_TRY(ctx_buffer(ctx, "sort_keys_out", (size_t)np * 4, (void**)&kout));
     PSX_TRY(ctx_buffer(ctx, "sort_idx_in", (size_t)np * 4, (void**)&vin));
     PSX_TRY(ctx_buffer(ctx, "sort_idx_out", (size_t)np * 4, (void**)&pout));
+    PSX_TRY(ctx_buffer(ctx, "sorted_particles", (size_t)np * sizeof(float4), (void**)&r));
     hipLaunchKernelGGL(k_tile_keys, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz,
                        np, N, kf_ks, clip_max, at, kin, vin);
     PSX_CHECK_LAUNCH();
@@ -341,13 +352,16 @@ static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t
     PSX_TRY(ctx_buffer(ctx, "sort_tmp", tmp_bytes > 0 ? tmp_bytes : 16, &tmp));
     PSX_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, pout, (size_t)np, 0, bits,
                                       ctx->stream));
-    perm = pout;
+    hipLaunchKernelGGL(k_gather_particles, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
+                       xyz, w, pout, np, clip_max, r);
+    PSX_CHECK_LAUNCH();
+    rec = r;
     keys = kout;
   }
-  if (!fft_x) return assign_lines_n<T, 0>(ctx, xyz, w, perm, keys, np, N, at, kf_ks, clip_max, scale, grid, false);
+  if (!fft_x) return assign_lines_n<T, 0>(ctx, rec, keys, np, N, at, kf_ks, scale, grid, false);
   switch (N) {
     PSX_FFT_SIZES(PSX_CASE_N)
-    default: return assign_lines_n<T, 0>(ctx, xyz, w, perm, keys, np, N, at, kf_ks, clip_max, scale, grid, true);
+    default: return assign_lines_n<T, 0>(ctx, rec, keys, np, N, at, kf_ks, scale, grid, true);
   }
 }
 #undef PSX_CASE_N
