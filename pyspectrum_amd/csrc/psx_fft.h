@@ -88,6 +88,26 @@ template <int NC> __device__ __forceinline__ int n_of(const FftShape& sh) {
   if constexpr (NC > 0) return NC; else return sh.N;
 }
 
+// Fill LDS from global memory with U independent loads in flight per thread
+// (a plain load/store loop waits for every load before its LDS store):
+// load(idx) returns element idx < n, store(idx, v) puts it into LDS.
+template <int U, typename V, typename Load, typename Store>
+__device__ __forceinline__ void batched_fill(int n, Load load, Store store) {
+  for (int i0 = threadIdx.x; i0 < n; i0 += FFT_THREADS * U) {
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int idx = i0 + u * FFT_THREADS;
+      if (idx < n) v[u] = load(idx);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int idx = i0 + u * FFT_THREADS;
+      if (idx < n) store(idx, v[u]);
+    }
+  }
+}
+
 // Copy the twiddle table into LDS (read by every radix stage).
 template <typename T>
 __device__ __forceinline__ void stage_twiddles(cplx<T>* twl, const cplx<T>* __restrict__ tw, int N) {

@@ -39,12 +39,25 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
   // offsets within a tile fit in 32 bits for N <= 1024; tl is a constant for
   // compile-time N, so the index divisions below are multiplications
   const int cs = (int)Cs, es = (int)Es;
-  for (int idx = threadIdx.x; idx < tl * N; idx += FFT_THREADS) {
-    int c, n;
-    if (es == 1) { c = idx / N; n = idx - c * N; }
-    else { n = idx / tl; c = idx - n * tl; }
-    if (c >= nl) continue;
-    lds[c * lstride + n] = in[base + (c * cs + n * es)];
+  if (es == 1) {
+    // x lines: the tile is one contiguous run of nl*N elements
+    batched_fill<8, cplx<T>>(
+        nl * N, [&](int idx) { return in[base + idx]; },
+        [&](int idx, cplx<T> v) {
+          const int c = idx / N;
+          lds[c * lstride + (idx - c * N)] = v;
+        });
+  } else {
+    batched_fill<8, cplx<T>>(
+        tl * N,
+        [&](int idx) {
+          const int n = idx / tl, c = idx - n * tl;
+          return c < nl ? in[base + (c * cs + n * es)] : mk(T(0), T(0));
+        },
+        [&](int idx, cplx<T> v) {
+          const int n = idx / tl, c = idx - n * tl;
+          if (c < nl) lds[c * lstride + n] = v;
+        });
   }
   cplx<T>* twl = lds + tl * lstride;
   stage_twiddles(twl, tw, N);
@@ -124,6 +137,15 @@ struct ShellDev {
 
 __device__ __forceinline__ int signed_f(int n, int N) { return n <= N / 2 ? n : n - N; }
 
+// Four consecutive values to global memory with vector stores (p is 4-element aligned).
+__device__ __forceinline__ void store4(float* p, const float (&o)[4]) {
+  *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+}
+__device__ __forceinline__ void store4(double* p, const double (&o)[4]) {
+  reinterpret_cast<double2*>(p)[0] = make_double2(o[0], o[1]);
+  reinterpret_cast<double2*>(p)[1] = make_double2(o[2], o[3]);
+}
+
 // z pass: tile (s, ky, kx0): NL kx columns of one ky row, masked to shell j.
 template <typename T, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restrict__ H,
@@ -139,17 +161,23 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
   const int j = sd.meta[4 * s], kxw = sd.meta[4 * s + 2];
   const int kys = signed_f(ky, N);
   constexpr int NL = TileCfg<T>::LINES;  // kx columns per tile (the plan's kt)
-  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
-    const int n = idx / NL, c = idx - n * NL;
-    const int kx = kx0 + c;
-    cplx<T> v = mk(T(0), T(0));
-    if (kx < Nh) {
-      const int kzs = signed_f(n, N);
-      const int64_t m = (int64_t)kx * kx + (int64_t)kys * kys + (int64_t)kzs * kzs;
-      if (shell_of_m(m, sd.step) == j) v = H[((int64_t)n * N + ky) * Nh + kx];
-    }
-    lds[c * lstride + n] = v;
-  }
+  batched_fill<8, cplx<T>>(
+      NL * N,
+      [&](int idx) {
+        const int n = idx / NL, c = idx - n * NL;
+        const int kx = kx0 + c;
+        cplx<T> v = mk(T(0), T(0));
+        if (kx < Nh) {
+          const int kzs = signed_f(n, N);
+          const int64_t m = (int64_t)kx * kx + (int64_t)kys * kys + (int64_t)kzs * kzs;
+          if (shell_of_m(m, sd.step) == j) v = H[((int64_t)n * N + ky) * Nh + kx];
+        }
+        return v;
+      },
+      [&](int idx, cplx<T> v) {
+        const int n = idx / NL, c = idx - n * NL;
+        lds[c * lstride + n] = v;
+      });
   cplx<T>* twl = lds + NL * lstride;
   stage_twiddles(twl, tw, N);
   __syncthreads();
@@ -178,13 +206,19 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
   const int64_t mm = sd.mmax[s];
   const int64_t base = sd.off[s] + (int64_t)z * N * kxw + kx0;
   constexpr int NL = TileCfg<T>::LINES;
-  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
-    const int n = idx / NL, c = idx - n * NL;
-    const int kys = signed_f(n, N);
-    cplx<T> v = mk(T(0), T(0));
-    if ((int64_t)kx0 * kx0 + (int64_t)kys * kys <= mm) v = T1[base + (int64_t)n * kxw + c];
-    lds[c * lstride + n] = v;
-  }
+  batched_fill<8, cplx<T>>(
+      NL * N,
+      [&](int idx) {
+        const int n = idx / NL, c = idx - n * NL;
+        const int kys = signed_f(n, N);
+        cplx<T> v = mk(T(0), T(0));
+        if ((int64_t)kx0 * kx0 + (int64_t)kys * kys <= mm) v = T1[base + (int64_t)n * kxw + c];
+        return v;
+      },
+      [&](int idx, cplx<T> v) {
+        const int n = idx / NL, c = idx - n * NL;
+        lds[c * lstride + n] = v;
+      });
   cplx<T>* twl = lds + NL * lstride;
   stage_twiddles(twl, tw, N);
   __syncthreads();
@@ -206,7 +240,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
                                                          T* __restrict__ D,
                                                          const Tile3* __restrict__ tiles,
                                                          ShellDev sd, FftShape sh,
-                                                         const cplx<T>* __restrict__ tw) {
+                                                         const cplx<T>* __restrict__ tw,
+                                                         bool vec4) {
   constexpr int NP = SHELL_X_PAIRS;
   constexpr int TPL = FFT_THREADS / NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -227,13 +262,24 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
   if (p < npair) {
     const cplx<T>* r1 = T2 + sd.off[s] + ((int64_t)z * N + y0 + 2 * p) * kxw;
     const cplx<T>* r2 = r1 + kxw;
-    for (int kx = lane; kx < kxl; kx += TPL) {
-      const cplx<T> a = r1[kx], b = r2[kx];
-      if (kx == 0 || kx == hN) {
-        L[kx] = mk(a.re, b.re);
-      } else {
-        L[kx] = mk(a.re - b.im, a.im + b.re);
-        L[N - kx] = mk(a.re + b.im, b.re - a.im);
+    constexpr int U = 4;
+    for (int k0 = lane; k0 < kxl; k0 += TPL * U) {
+      cplx<T> a[U], b[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kx = k0 + u * TPL;
+        if (kx < kxl) { a[u] = r1[kx]; b[u] = r2[kx]; }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kx = k0 + u * TPL;
+        if (kx >= kxl) continue;
+        if (kx == 0 || kx == hN) {
+          L[kx] = mk(a[u].re, b[u].re);
+        } else {
+          L[kx] = mk(a[u].re - b[u].im, a[u].im + b[u].re);
+          L[N - kx] = mk(a[u].re + b[u].im, b[u].re - a[u].im);
+        }
       }
     }
   }
@@ -242,10 +288,23 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
   // rows y0 .. y0 + 2 npair - 1 are contiguous in D: one coalesced stream
   const int64_t G = (int64_t)N * N * N;
   T* dst = D + (int64_t)s * G + ((int64_t)z * N + y0) * N;
-  for (int idx = threadIdx.x; idx < 2 * npair * N; idx += FFT_THREADS) {
-    const int r = idx / N, n = idx - r * N;
-    const cplx<T> v = lds[(r >> 1) * lstride + n];
-    dst[idx] = (r & 1) ? v.im : v.re;
+  if (vec4) {
+    // N % 4 == 0 and D 16-byte aligned: four consecutive values of one row
+    // per thread, 16/32-byte stores
+    for (int i4 = threadIdx.x; i4 < 2 * npair * N / 4; i4 += FFT_THREADS) {
+      const int idx = 4 * i4, r = idx / N, n = idx - r * N;
+      const cplx<T>* src = lds + (r >> 1) * lstride + n;
+      T o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[e] = (r & 1) ? src[e].im : src[e].re;
+      store4(dst + idx, o);
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < 2 * npair * N; idx += FFT_THREADS) {
+      const int r = idx / N, n = idx - r * N;
+      const cplx<T> v = lds[(r >> 1) * lstride + n];
+      dst[idx] = (r & 1) ? v.im : v.re;
+    }
   }
 }
 
@@ -272,7 +331,8 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
   if (sp->nx_tiles > 0) {
     PSX_TRY(set_lds(k_shell_x<T, NC>, ldsx));
     hipLaunchKernelGGL((k_shell_x<T, NC>), dim3(sp->nx_tiles), dim3(FFT_THREADS), ldsx, ctx->stream,
-                       (const cplx<T>*)t2, dout, sp->d_xtiles, sd, p->shape, tw);
+                       (const cplx<T>*)t2, dout, sp->d_xtiles, sd, p->shape, tw,
+                       N % 4 == 0 && ((uintptr_t)dout % 16) == 0);
     PSX_CHECK_LAUNCH();
   }
   return PSX_OK;

@@ -193,9 +193,21 @@ __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
   __syncthreads();
   const int64_t total = pre[nsrc];
   int s = 0;
-  for (int64_t q = threadIdx.x; q < total; q += FFT_THREADS) {
-    while (q >= pre[s + 1]) ++s;
-    const float4 r = rec[sbeg[s] + (q - pre[s])];
+  constexpr int U = 4;  // particle records loaded ahead of their use
+  for (int64_t q0 = threadIdx.x; q0 < total; q0 += FFT_THREADS * U) {
+    float4 rq[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = q0 + (int64_t)u * FFT_THREADS;
+      if (q < total) {
+        while (q >= pre[s + 1]) ++s;
+        rq[u] = rec[sbeg[s] + (q - pre[s])];
+      }
+    }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (q0 + (int64_t)u * FFT_THREADS >= total) break;
+    const float4 r = rq[u];
     const float p[3] = {r.x, r.y, r.z};
     const T we = (T)r.w;
 #pragma unroll
@@ -239,6 +251,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
         }
       }
     }
+  }
   }
   __syncthreads();
   const double inv = 1.0 / scale;
