@@ -82,6 +82,39 @@ def test_assign_quad_matches_oracle(prec):
     assert _rel(out.real, A) < tol and _rel(out.imag, B) < tol
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["one_cell", "single", "periodic_edges"])
+def test_assign_edge_cases(prec, case):
+    """Dense and degenerate inputs of the tile assignment: 50000 points inside
+    one cell (every stencil lands in the same 4^3 block, LDS atomics collide),
+    a single particle, and particles on the box faces (stencils that wrap
+    around all three axes into tiles at the far side)."""
+    _say("assign edge", case, prec)
+    N, Lbox = 32, 400.0
+    rng = np.random.default_rng(9)
+    if case == "one_cell":
+        xyz = 100.0 + rng.uniform(0, Lbox / N, size=(3, 50000))
+    elif case == "single":
+        xyz = np.array([[123.4], [5.6], [399.9]])
+    else:
+        xyz = rng.uniform(0, Lbox, size=(3, 2000))
+        xyz[:, :1000] = rng.choice([0.0, Lbox - 1e-3, Lbox / N * 0.5], size=(3, 1000))
+    w = rng.uniform(0.5, 1.5, size=xyz.shape[1])
+    xyz32 = S.clip_positions(xyz, Lbox)
+    kf_ks = np.float32(N / Lbox)
+    A, B = S.assign_quad(xyz32, w.astype(np.float32).astype(np.float64), N, kf_ks)
+    x = torch.from_numpy(np.ascontiguousarray(xyz.astype(np.float32))).to(DEV)
+    wd = torch.from_numpy(w.astype(np.float32)).to(DEV)
+    g = torch.empty((N, N, N), dtype=_cdt(prec), device=DEV)
+    L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N, float(kf_ks),
+           float(np.float32(Lbox * (1 - 1e-6))), float(np.abs(w).sum()), L.ptr(g), L.PREC[prec])
+    out = g.cpu().numpy().transpose(2, 1, 0)
+    tol = 1e-11 if prec == "fp64" else 5e-7
+    assert _rel(out.real, A) < tol and _rel(out.imag, B) < tol
+    # mass: every particle deposits 216 w into each grid
+    assert abs(out.real.sum() - 216 * w.astype(np.float32).sum()) < 1e-4 * 216 * w.sum()
+
+
 def test_assign_is_deterministic():
     _say("assign determinism")
     N, Lbox = 64, 1000.0
