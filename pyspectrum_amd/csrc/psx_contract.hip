@@ -106,26 +106,45 @@ __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
   return __builtin_bit_cast(unsigned, __builtin_convertvector(x, bf2v));
 }
 
+// Residuals of a packed bf16 pair hb = (bf16(x0), bf16(x1)): x0 - hi0 and
+// x1 - hi1, each one v_dot2_f32_bf16 (hb . (-1, 0) + x0, hb . (0, -1) + x1).
+// The difference of a float and its nearest bf16 is exactly representable,
+// so the single rounding of the dot product leaves it exact.
+// The (-1, 0) and (0, -1) pairs are passed in registers: an inline constant
+// for a packed-bf16 operand is not the same bit pattern on every path.
+__device__ __forceinline__ void bf16_residuals(unsigned hb, float x0, float x1, unsigned neg_lo,
+                                               unsigned neg_hi, float& r0, float& r1) {
+  const bf2v h = __builtin_bit_cast(bf2v, hb);
+  r0 = __builtin_amdgcn_fdot2_f32_bf16(h, __builtin_bit_cast(bf2v, neg_lo), x0, false);
+  r1 = __builtin_amdgcn_fdot2_f32_bf16(h, __builtin_bit_cast(bf2v, neg_hi), x1, false);
+}
+
+// A value the compiler must keep in a register (no constant folding).
+__device__ __forceinline__ unsigned in_register(unsigned v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // Products p = a*b of 8 point pairs, split as p = hi + lo: hi = bf16(p), lo =
-// bf16(a*b - hi) with the residual formed exactly by one FMA.  Per value:
-// a multiply, half a pack, a shift or mask back to float, an FMA, half a pack.
+// bf16(p - hi).  Per value: a multiply, half a pack, one dot2 for the
+// residual, half a pack.
 __device__ __forceinline__ void product_split_bf16(const f4v& a0, const f4v& a1, const f4v& b0,
-                                                   const f4v& b1, bf8v& hi, bf8v& lo) {
-  float a[8], b[8];
+                                                   const f4v& b1, unsigned neg_lo, unsigned neg_hi,
+                                                   bf8v& hi, bf8v& lo) {
+  float x[8];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    a[e] = a0[e]; a[4 + e] = a1[e];
-    b[e] = b0[e]; b[4 + e] = b1[e];
+    x[e] = a0[e] * b0[e];
+    x[4 + e] = a1[e] * b1[e];
   }
   u4v h, l;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    const float x0 = a[2 * p] * b[2 * p], x1 = a[2 * p + 1] * b[2 * p + 1];
-    const unsigned hb = pack_bf16(x0, x1);
-    const float h0 = __builtin_bit_cast(float, hb << 16);
-    const float h1 = __builtin_bit_cast(float, hb & 0xffff0000u);
+    const unsigned hb = pack_bf16(x[2 * p], x[2 * p + 1]);
+    float r0, r1;
+    bf16_residuals(hb, x[2 * p], x[2 * p + 1], neg_lo, neg_hi, r0, r1);
     h[p] = hb;
-    l[p] = pack_bf16(fmaf(a[2 * p], b[2 * p], -h0), fmaf(a[2 * p + 1], b[2 * p + 1], -h1));
+    l[p] = pack_bf16(r0, r1);
   }
   hi = __builtin_bit_cast(bf8v, h);
   lo = __builtin_bit_cast(bf8v, l);
@@ -165,6 +184,8 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
   const int kq = lane >> 4;    // k group
   const int gw = (int)blockIdx.y * CT_WAVES + wave;
   const int kbase = BF ? kq * 8 : kq * KQ;
+  // bf16 pairs (lo, hi) = (-1, 0) and (0, -1) for the split residuals
+  const unsigned neg_lo = in_register(0x0000BF80u), neg_hi = in_register(0xBF800000u);
 
   // wave-uniform tile list (pair tiles fill q = 0 .. nq-1)
   int nq = 0, msk[MAXPT], oi[MAXPT], oj[MAXPT];
@@ -262,9 +283,10 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
           for (int p = 0; p < 2; ++p) {
             const float x0 = (float)pre[u][2 * p], x1 = (float)pre[u][2 * p + 1];
             const unsigned hb = pack_bf16(x0, x1);
+            float r0, r1;
+            bf16_residuals(hb, x0, x1, neg_lo, neg_hi, r0, r1);
             h[p] = hb;
-            l[p] = pack_bf16(x0 - __builtin_bit_cast(float, hb << 16),
-                             x1 - __builtin_bit_cast(float, hb & 0xffff0000u));
+            l[p] = pack_bf16(r0, r1);
           }
           *reinterpret_cast<u2v*>(ldh + s * LDWH + k) = h;
           *reinterpret_cast<u2v*>(ldl + s * LDWH + k) = l;
@@ -292,7 +314,7 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
           const f4v* pa = reinterpret_cast<const f4v*>(lds + oi[q] + k0);
           const f4v* pb = reinterpret_cast<const f4v*>(lds + oj[q] + k0);
           bf8v ah, al;
-          product_split_bf16(pa[0], pa[1], pb[0], pb[1], ah, al);
+          product_split_bf16(pa[0], pa[1], pb[0], pb[1], neg_lo, neg_hi, ah, al);
 #pragma unroll
           for (int t = 0; t < NLT; ++t) {
             if ((msk[q] >> t) & 1) {
