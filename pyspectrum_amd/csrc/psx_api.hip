@@ -214,6 +214,81 @@ static void triangle_list(int step, int ncut, int nmax, std::vector<int>* ijl) {
 }
 
 // ------------------------------------------------------- contract plan ----
+// Which shell pair reads each triangle.  B is symmetric in its three shells,
+// so a triangle (i >= j >= l) is Sum_x P D with P = d_a d_b for any of its
+// pairs (i, j), (i, l), (j, l) and D the remaining shell.  Forming a pair's
+// products is the costly part of the contraction, so the pairs are a greedy
+// set cover (repeatedly the pair that reads most uncovered triangles, ties to
+// the smallest key); each triangle then moves to the chosen pair whose
+// (pair, 16-column tile) cell reads most triangles, which keeps the number
+// of 16x16 output blocks down.  At Nmax = 40, step 3 this is 339 pairs for
+// the 440 pairs (i, j) of the reference loop, with as many blocks.
+// Pair key = a * (nmax + 1) + b, a >= b.  Deterministic.
+static void triangle_cover(const std::vector<int>& ijl, int nmax, int j0, std::vector<int>* tpair,
+                           std::vector<int>* tshell) {
+  const int ntri = (int)(ijl.size() / 3), nsh = nmax + 1, nkeys = nsh * nsh;
+  // the distinct (pair, remaining shell) readings of triangle t
+  auto readings = [&](int t, int (&key)[3], int (&rem)[3]) {
+    const int i = ijl[3 * t], j = ijl[3 * t + 1], l = ijl[3 * t + 2];
+    int n = 0;
+    auto add = [&](int a, int b, int c) {
+      const int k = a * nsh + b;
+      for (int m = 0; m < n; ++m)
+        if (key[m] == k) return;
+      key[n] = k; rem[n] = c; ++n;
+    };
+    add(i, j, l); add(i, l, j); add(j, l, i);
+    return n;
+  };
+  int key[3], rem[3];
+  std::vector<int> cnt(nkeys, 0);
+  std::vector<std::vector<int>> tris_of(nkeys);
+  for (int t = 0; t < ntri; ++t) {
+    const int n = readings(t, key, rem);
+    for (int m = 0; m < n; ++m) { cnt[key[m]]++; tris_of[key[m]].push_back(t); }
+  }
+  std::vector<char> covered(ntri, 0), chosen(nkeys, 0);
+  tpair->assign(ntri, -1);
+  tshell->assign(ntri, -1);
+  for (;;) {
+    int best = -1;
+    for (int k = 0; k < nkeys; ++k)
+      if (cnt[k] > 0 && (best < 0 || cnt[k] > cnt[best])) best = k;
+    if (best < 0) break;
+    chosen[best] = 1;
+    for (int t : tris_of[best]) {
+      if (covered[t]) continue;
+      covered[t] = 1;
+      const int n = readings(t, key, rem);
+      for (int m = 0; m < n; ++m) {
+        cnt[key[m]]--;
+        if (key[m] == best) { (*tpair)[t] = best; (*tshell)[t] = rem[m]; }
+      }
+    }
+  }
+  // move triangles to the fullest (pair, column tile) cell among the chosen pairs
+  auto cell = [&](int k, int s) { return k * 4 + (s - j0) / 16; };
+  std::vector<int> fill(nkeys * 4, 0);
+  for (int t = 0; t < ntri; ++t) fill[cell((*tpair)[t], (*tshell)[t])]++;
+  for (int pass = 0; pass < 5; ++pass) {
+    for (int t = 0; t < ntri; ++t) {
+      const int cur = cell((*tpair)[t], (*tshell)[t]);
+      const int n = readings(t, key, rem);
+      int bk = (*tpair)[t], bs = (*tshell)[t], bscore = fill[cur] - 1;
+      for (int m = 0; m < n; ++m) {
+        if (!chosen[key[m]] || key[m] == (*tpair)[t]) continue;
+        const int score = fill[cell(key[m], rem[m])];
+        if (score > bscore) { bk = key[m]; bs = rem[m]; bscore = score; }
+      }
+      if (bk != (*tpair)[t]) {
+        fill[cur]--;
+        fill[cell(bk, bs)]++;
+        (*tpair)[t] = bk; (*tshell)[t] = bs;
+      }
+    }
+  }
+}
+
 int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
                       const ContractPlan** out);
 
@@ -230,28 +305,47 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
   cp->nlt = (int)cdiv(ns, 16);
   if (cp->nlt > 4)
     return set_error(PSX_ERR_UNSUPPORTED, "at most 64 shells (Nmax - Ncut//step + 1 <= 64)");
-  // pairs (i, j), i >= j, i <= 2j, sorted by (j, i)
-  std::map<std::pair<int, int>, int> pidx;
-  std::vector<int> pair_s;
-  for (int j = j0; j <= nmax; ++j)
-    for (int i = j; i <= std::min(2 * j, nmax); ++i) {
-      pidx[{i, j}] = (int)pidx.size();
-      pair_s.push_back(i - j0);
-      pair_s.push_back(j - j0);
-    }
-  cp->npairs = (int)pidx.size();
-  cp->npt = (int)cdiv(cp->npairs, 16);
-  while ((int)pair_s.size() < cp->npt * 32) pair_s.push_back(-1);
   std::vector<int> ijl;
   triangle_list(step, ncut, nmax, &ijl);
   cp->ntri = (int)(ijl.size() / 3);
+  std::vector<int> tpair, tshell;
+  triangle_cover(ijl, nmax, j0, &tpair, &tshell);
+  // pair tiles: pairs grouped by their set of column tiles, whole tiles of one
+  // set first, the remainders sorted by (tiles used, set) and cut into tiles
+  std::map<int, int> pmask;  // pair key -> column tiles it reads
+  for (int t = 0; t < cp->ntri; ++t) pmask[tpair[t]] |= 1 << ((tshell[t] - j0) / 16);
+  std::map<int, std::vector<int>> by_mask;
+  for (const auto& pm : pmask) by_mask[pm.second].push_back(pm.first);
+  std::vector<int> prow;  // pair key of each row, -1 = padding
+  std::vector<std::pair<int, int>> rest;
+  for (const auto& bm : by_mask) {
+    const size_t whole = bm.second.size() / 16 * 16;
+    prow.insert(prow.end(), bm.second.begin(), bm.second.begin() + whole);
+    for (size_t k = whole; k < bm.second.size(); ++k) rest.push_back({bm.first, bm.second[k]});
+  }
+  std::stable_sort(rest.begin(), rest.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+    const int pa = __builtin_popcount(a.first), pb = __builtin_popcount(b.first);
+    return pa != pb ? pa < pb : a.first < b.first;
+  });
+  for (const auto& r : rest) prow.push_back(r.second);
+  cp->npairs = (int)prow.size();
+  cp->npt = (int)cdiv(cp->npairs, 16);
+  prow.resize((size_t)cp->npt * 16, -1);
+  const int nsh = nmax + 1;
+  std::vector<int> pair_s(2 * prow.size(), -1);
+  std::map<int, int> prow_of;
+  for (size_t r = 0; r < prow.size(); ++r) {
+    if (prow[r] < 0) continue;
+    pair_s[2 * r] = prow[r] / nsh - j0;
+    pair_s[2 * r + 1] = prow[r] % nsh - j0;
+    prow_of[prow[r]] = (int)r;
+  }
   std::vector<int> ptmask(cp->npt, 0);
   std::vector<int> tpt(cp->ntri), trow(cp->ntri), tlt(cp->ntri), tcol(cp->ntri);
   for (int t = 0; t < cp->ntri; ++t) {
-    const int i = ijl[3 * t], j = ijl[3 * t + 1], l = ijl[3 * t + 2];
-    const int p = pidx.at({i, j});
+    const int p = prow_of.at(tpair[t]);
     tpt[t] = p / 16; trow[t] = p % 16;
-    tlt[t] = (l - j0) / 16; tcol[t] = (l - j0) % 16;
+    tlt[t] = (tshell[t] - j0) / 16; tcol[t] = (tshell[t] - j0) % 16;
     ptmask[tpt[t]] |= 1 << tlt[t];
   }
   cp->nblk = 0;
