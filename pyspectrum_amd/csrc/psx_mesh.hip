@@ -3,8 +3,6 @@
 // full field conversions.
 #include <cstring>
 
-#include <rocprim/rocprim.hpp>
-
 #include "psx_ctx.h"
 
 namespace psx {
@@ -90,29 +88,70 @@ static AssignTiling assign_tiling(int N) {
   return a;
 }
 
-// Sort key of a particle: the assignment tile of its grid-A base cell.
-__global__ void k_tile_keys(const float* __restrict__ xyz, int64_t np, int N, float kf_ks,
-                            float clip_max, AssignTiling at, unsigned* __restrict__ keys,
-                            unsigned* __restrict__ idx) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
+// Particles are ordered by assignment tile with a counting sort: count per
+// tile (the atomic's return value is the particle's rank in its tile), an
+// exclusive scan of the counts, and a scatter of (x, y, z, w) records to
+// offset + rank.  The order inside a tile depends on the atomics, which does
+// not matter: the tile's workgroup adds integer contributions.
+__device__ __forceinline__ unsigned tile_key(float y, float z, int N, float kf_ks, AssignTiling at) {
   int cy, cz;
-  pcs_coord(clip_pos(xyz[np + i], clip_max), kf_ks, false, N, &cy);
-  pcs_coord(clip_pos(xyz[2 * np + i], clip_max), kf_ks, false, N, &cz);
-  keys[i] = (unsigned)(cz / at.tz) * (unsigned)at.nty + (unsigned)(cy / at.ty);
-  idx[i] = (unsigned)i;
+  pcs_coord(y, kf_ks, false, N, &cy);
+  pcs_coord(z, kf_ks, false, N, &cz);
+  return (unsigned)(cz / at.tz) * (unsigned)at.nty + (unsigned)(cy / at.ty);
 }
 
-// Particles in sorted order as (x, y, z, w) records, positions already
-// clipped, so the tile kernel reads contiguous 16-byte records.
-__global__ void k_gather_particles(const float* __restrict__ xyz, const float* __restrict__ w,
-                                   const unsigned* __restrict__ perm, int64_t np, float clip_max,
-                                   float4* __restrict__ rec) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  const int64_t i = (int64_t)perm[t];
-  rec[t] = make_float4(clip_pos(xyz[i], clip_max), clip_pos(xyz[np + i], clip_max),
-                       clip_pos(xyz[2 * np + i], clip_max), w ? w[i] : 1.0f);
+__global__ void k_fill_u32(unsigned* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void k_tile_count(const float* __restrict__ xyz, int64_t np, int N, float kf_ks,
+                             float clip_max, AssignTiling at, unsigned* __restrict__ count,
+                             unsigned* __restrict__ key, unsigned* __restrict__ rank) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const unsigned k = tile_key(clip_pos(xyz[np + i], clip_max), clip_pos(xyz[2 * np + i], clip_max),
+                              N, kf_ks, at);
+  key[i] = k;
+  rank[i] = atomicAdd(count + k, 1u);
+}
+
+// Exclusive scan of n counts into off[0..n] (off[n] = total), one workgroup:
+// thread t sums a contiguous segment, the segment sums are scanned in LDS.
+constexpr int SCAN_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const unsigned* __restrict__ count,
+                                                            int n, unsigned* __restrict__ off) {
+  __shared__ unsigned part[SCAN_THREADS];
+  const int t = threadIdx.x;
+  const int seg = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+  const int b = min(n, t * seg), e = min(n, b + seg);
+  unsigned s = 0;
+  for (int i = b; i < e; ++i) s += count[i];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < SCAN_THREADS; d <<= 1) {
+    const unsigned v = t >= d ? part[t - d] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned run = part[t] - s;  // exclusive prefix of this segment
+  for (int i = b; i < e; ++i) {
+    off[i] = run;
+    run += count[i];
+  }
+  if (t == SCAN_THREADS - 1) off[n] = part[t];
+}
+
+__global__ void k_tile_scatter(const float* __restrict__ xyz, const float* __restrict__ w,
+                               const unsigned* __restrict__ key, const unsigned* __restrict__ rank,
+                               const unsigned* __restrict__ off, int64_t np, float clip_max,
+                               float4* __restrict__ rec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  rec[(int64_t)off[key[i]] + rank[i]] =
+      make_float4(clip_pos(xyz[i], clip_max), clip_pos(xyz[np + i], clip_max),
+                  clip_pos(xyz[2 * np + i], clip_max), w ? w[i] : 1.0f);
 }
 
 // Tiles (width tw) along one axis holding the base cells c in [t0-3, t1]
@@ -131,16 +170,6 @@ __device__ int source_tiles(int t0, int t1, int N, int tw, int* out) {
   return n;
 }
 
-__device__ int64_t lower_bound_u32(const unsigned* __restrict__ a, int64_t n, unsigned key) {
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (a[mid] < key) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
 __device__ __forceinline__ int wrap1(int c, int N) {
   // c in [-1, 2N): one correction
   return c < 0 ? c + N : (c >= N ? c - N : c);
@@ -150,7 +179,7 @@ __device__ __forceinline__ int wrap1(int c, int N) {
 // same memory then holds the complex lines (row stride N+1) and the twiddles.
 template <typename T, int NC, bool FFTX>
 __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
-    const float4* __restrict__ rec, const unsigned* __restrict__ keys, int64_t np, FftShape sh,
+    const float4* __restrict__ rec, const unsigned* __restrict__ tile_off, int64_t np, FftShape sh,
     AssignTiling at, float kf_ks, double scale, const cplx<T>* __restrict__ tw,
     cplx<T>* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -178,8 +207,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
   const int nsrc = nsrc_s;
   if ((int)threadIdx.x < nsrc) {
     const unsigned key = (unsigned)src[threadIdx.x];
-    sbeg[threadIdx.x] = lower_bound_u32(keys, np, key);
-    pre[threadIdx.x + 1] = lower_bound_u32(keys, np, key + 1u);  // range end for now
+    // records of tile key: [tile_off[key], tile_off[key + 1])
+    sbeg[threadIdx.x] = np > 0 ? (int64_t)tile_off[key] : 0;
+    pre[threadIdx.x + 1] = np > 0 ? (int64_t)tile_off[key + 1] : 0;  // range end for now
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -307,7 +337,7 @@ int fixed_point_exponent(double wsum_abs) {
 }
 
 template <typename T, int NC>
-static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* keys, int64_t np, int N,
+static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off, int64_t np, int N,
                           AssignTiling at, float kf_ks, double scale, cplx<T>* grid, bool fft_x) {
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
@@ -319,12 +349,12 @@ static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* keys,
     const size_t lds = acc_bytes > fft_bytes ? acc_bytes : fft_bytes;
     auto kern = k_assign_lines<T, NC, true>;
     PSX_TRY(set_lds(kern, lds));
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, rec, keys, np,
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, rec, off, np,
                        p->shape, at, kf_ks, scale, plan_tw<T>(p), grid);
   } else {
     auto kern = k_assign_lines<T, 0, false>;
     PSX_TRY(set_lds(kern, acc_bytes));
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, keys,
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, off,
                        np, p->shape, at, kf_ks, scale, plan_tw<T>(p), grid);
   }
   PSX_CHECK_LAUNCH();
@@ -333,7 +363,7 @@ static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* keys,
 
 #define PSX_CASE_N(n)                                                                         \
   case n:                                                                                     \
-    return assign_lines_n<T, n>(ctx, rec, keys, np, N, at, kf_ks, scale, grid, fft_x);
+    return assign_lines_n<T, n>(ctx, rec, off, np, N, at, kf_ks, scale, grid, fft_x);
 
 template <typename T>
 static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
@@ -341,40 +371,37 @@ static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t
   PSX_REQUIRE(np < (int64_t)1 << 32, "more than 2^32 - 1 particles");
   const AssignTiling at = assign_tiling<T>(N);
   const float4* rec = nullptr;
-  const unsigned* keys = nullptr;
+  const unsigned* off = nullptr;
   if (np > 0) {
-    // order the particles by assignment tile (rocPRIM radix sort of 32-bit
-    // keys), then gather them into sorted (x, y, z, w) records
-    unsigned *kin, *kout, *vin, *pout;
-    float4* r;
-    PSX_TRY(ctx_buffer(ctx, "sort_keys_in", (size_t)np * 4, (void**)&kin));
-    PSX_TRY(ctx_buffer(ctx, "sort_keys_out", (size_t)np * 4, (void**)&kout));
-    PSX_TRY(ctx_buffer(ctx, "sort_idx_in", (size_t)np * 4, (void**)&vin));
-    PSX_TRY(ctx_buffer(ctx, "sort_idx_out", (size_t)np * 4, (void**)&pout));
-    PSX_TRY(ctx_buffer(ctx, "sorted_particles", (size_t)np * sizeof(float4), (void**)&r));
-    hipLaunchKernelGGL(k_tile_keys, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz,
-                       np, N, kf_ks, clip_max, at, kin, vin);
-    PSX_CHECK_LAUNCH();
+    // counting sort by assignment tile into (x, y, z, w) records
     const int64_t ntiles = (int64_t)at.nty * at.ntz;
-    int bits = 1;
-    while (((int64_t)1 << bits) < ntiles) ++bits;
-    size_t tmp_bytes = 0;
-    PSX_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, pout, (size_t)np, 0, bits,
-                                      ctx->stream));
-    void* tmp;
-    PSX_TRY(ctx_buffer(ctx, "sort_tmp", tmp_bytes > 0 ? tmp_bytes : 16, &tmp));
-    PSX_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, pout, (size_t)np, 0, bits,
-                                      ctx->stream));
-    hipLaunchKernelGGL(k_gather_particles, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
-                       xyz, w, pout, np, clip_max, r);
+    unsigned *count, *key, *rank, *o;
+    float4* r;
+    PSX_TRY(ctx_buffer(ctx, "tile_count", (size_t)ntiles * 4, (void**)&count));
+    PSX_TRY(ctx_buffer(ctx, "tile_off", (size_t)(ntiles + 1) * 4, (void**)&o));
+    PSX_TRY(ctx_buffer(ctx, "particle_tile", (size_t)np * 4, (void**)&key));
+    PSX_TRY(ctx_buffer(ctx, "particle_rank", (size_t)np * 4, (void**)&rank));
+    PSX_TRY(ctx_buffer(ctx, "sorted_particles", (size_t)np * sizeof(float4), (void**)&r));
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)cdiv(ntiles, 256)), dim3(256), 0, ctx->stream,
+                       count, ntiles);
+    PSX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_tile_count, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz,
+                       np, N, kf_ks, clip_max, at, count, key, rank);
+    PSX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream,
+                       (const unsigned*)count, (int)ntiles, o);
+    PSX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_tile_scatter, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
+                       xyz, w, (const unsigned*)key, (const unsigned*)rank, (const unsigned*)o, np,
+                       clip_max, r);
     PSX_CHECK_LAUNCH();
     rec = r;
-    keys = kout;
+    off = o;
   }
-  if (!fft_x) return assign_lines_n<T, 0>(ctx, rec, keys, np, N, at, kf_ks, scale, grid, false);
+  if (!fft_x) return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, kf_ks, scale, grid, false);
   switch (N) {
     PSX_FFT_SIZES(PSX_CASE_N)
-    default: return assign_lines_n<T, 0>(ctx, rec, keys, np, N, at, kf_ks, scale, grid, true);
+    default: return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, kf_ks, scale, grid, true);
   }
 }
 #undef PSX_CASE_N

@@ -36,7 +36,7 @@ def footprint_bytes(Ngrid, npart, Nmax=40, step=3, Ncut=3, precision="fp32"):
     kx = min(Ngrid // 2 + 1, step * (Nmax + 1) + 16)
     grids = 2 * r * G + 2 * r * G / 2                      # packed FFT grid, half field
     shells = ns * G * r + 2 * 2 * r * Ngrid * Ngrid * kx * ns / 2   # D + T1/T2 (average kx)
-    parts = 32.0 * npart                                   # sort keys/indices, sorted records
+    parts = 24.0 * npart                                   # tile keys and ranks, sorted records
     return int(grids + shells + parts + (64 << 20))
 
 
