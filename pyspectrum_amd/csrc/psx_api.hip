@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "psx_ctx.h"
@@ -72,6 +73,18 @@ int ctx_buffer(psx_ctx* ctx, const char* name, size_t bytes, void** out) {
   }
   *out = b.ptr;
   return PSX_OK;
+}
+
+int lds_attribute_needed(const void* kernel, size_t bytes) {
+  static std::mutex m;
+  static std::map<std::pair<int, const void*>, size_t> set;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  std::lock_guard<std::mutex> g(m);
+  size_t& have = set[{dev, kernel}];
+  if (have >= bytes) return 0;
+  have = bytes;
+  return 1;
 }
 
 int ctx_alloc_owned(psx_ctx* ctx, size_t bytes, void** out) {

@@ -97,10 +97,13 @@ template <typename T> inline const cplx<T>* plan_tw(const FftPlan* p);
 template <> inline const cplx<float>* plan_tw<float>(const FftPlan* p) { return (const cplx<float>*)p->tw32; }
 template <> inline const cplx<double>* plan_tw<double>(const FftPlan* p) { return (const cplx<double>*)p->tw64; }
 
-// Opt a kernel in to more than 64 KB of dynamic LDS.
+// Opt a kernel in to more than 64 KB of dynamic LDS.  The attribute is set
+// once per (device, kernel) and size increase, so a warmed-up pipeline makes
+// no runtime calls besides launches (it can be captured in a HIP graph).
+int lds_attribute_needed(const void* kernel, size_t bytes);
 template <typename K>
 inline int set_lds(K kernel, size_t bytes) {
-  if (bytes > 65536) {
+  if (bytes > 65536 && lds_attribute_needed((const void*)kernel, bytes)) {
     PSX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)bytes));
   }

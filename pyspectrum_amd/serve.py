@@ -58,22 +58,32 @@ class Request:
 
 
 class Slot:
-    """A worker: its own library context (workspace, plans) and stream."""
+    """A worker: its own library context (workspace, plans) and stream, and
+    optionally one captured B(k) graph per box shape (graphs.BkGraph)."""
 
     def __init__(self, device):
         self.device = device
         self.ctx = L.Context(device)
+        self.graphs = {}
         with torch.cuda.device(device):
             self.stream = torch.cuda.Stream(device=device)
+
+    def add_graph(self, key):
+        from .graphs import BkGraph
+        if key not in self.graphs:
+            N, step, Ncut, Nmax, prec = key
+            self.graphs[key] = BkGraph(N, step, Ncut, Nmax, prec, device=self.device)
 
     def run(self, req):
         dev = torch.device("cuda", self.device)
         prec = L.PREC[req.precision]
         N, n = req.Ngrid, req.xyz.shape[1]
+        graph = self.graphs.get((N, req.step, req.Ncut, req.Nmax, req.precision))
         with torch.cuda.stream(self.stream):
             h = self.ctx.handle  # binds this slot's stream
             cd = torch.complex128 if req.precision == "fp64" else torch.complex64
-            half = torch.empty((N, N, N // 2 + 1), dtype=cd, device=dev)
+            half = graph.half if graph is not None else torch.empty((N, N, N // 2 + 1), dtype=cd,
+                                                                    device=dev)
             L.call("psx_fft_periodic", h, L.ptr(req.xyz), None, n, N, float(req.Lbox), float(n),
                    float(n), L.ptr(half), prec)
             pk = P._pk_pbox_rsd(half, 2, req.Lbox, N // 2, req.Nmubin, N, prec, self.device,
@@ -83,10 +93,14 @@ class Slot:
                                      device=self.device)
             tc = P._tri_counts(counts, ijl, N)
             ntri = len(ijl)
-            braw = torch.empty(max(ntri, 1), dtype=torch.float64, device=dev)
-            p0k = torch.empty(req.Nmax, dtype=torch.float64, device=dev)
-            L.call("psx_bk_raw", self.ctx.handle, L.ptr(half), N, req.step, req.Ncut, req.Nmax,
-                   L.ptr(braw), L.ptr(p0k), prec)
+            if graph is not None:
+                graph.replay()  # shell fields, contraction, p0k on this stream
+                braw, p0k = graph.braw, graph.p0k
+            else:
+                braw = torch.empty(max(ntri, 1), dtype=torch.float64, device=dev)
+                p0k = torch.empty(req.Nmax, dtype=torch.float64, device=dev)
+                L.call("psx_bk_raw", self.ctx.handle, L.ptr(half), N, req.step, req.Ncut,
+                       req.Nmax, L.ptr(braw), L.ptr(p0k), prec)
             nbar = n / req.Lbox ** 3
             kf = 2 * np.pi / req.Lbox
             cnt = torch.from_numpy(tc).to(dev)
@@ -106,14 +120,26 @@ class Slot:
 class Server:
     """Admission by slots and HBM budget; requests run concurrently on slots."""
 
-    def __init__(self, nslots=4, hbm_budget=None, device=0):
+    def __init__(self, nslots=4, hbm_budget=None, device=0, use_graphs=False):
         self.device = device
         free, total = torch.cuda.mem_get_info(device)
         self.budget = int(hbm_budget if hbm_budget is not None else 0.8 * free)
+        self._fixed_budget = hbm_budget is not None
+        self.use_graphs = use_graphs
+        self.graph_build_s = 0.0
         self.slots = [Slot(device) for _ in range(nslots)]
         self._lock = threading.Condition()
         self._inflight_bytes = 0
         self._free_slots = list(range(nslots))
+
+    def close(self):
+        """Release the slots' graphs and library contexts."""
+        for slot in self.slots:
+            for g in slot.graphs.values():
+                g.close()
+            slot.graphs.clear()
+            slot.ctx.close()
+        torch.cuda.empty_cache()
 
     def _admit(self, need):
         with self._lock:
@@ -136,6 +162,19 @@ class Server:
         # N_tri caches are computed (or read) before the clock starts
         for key in sorted({(r.Ngrid, r.Nmax, r.Ncut, r.step) for r in reqs}):
             P._counts_Bk123(Ngrid=key[0], Nmax=key[1], Ncut=key[2], step=key[3], device=self.device)
+        if self.use_graphs:
+            # one captured B(k) graph per slot and box shape, built before the
+            # clock; the budget then covers what is left
+            tg = time.perf_counter()
+            shapes = sorted({(r.Ngrid, r.step, r.Ncut, r.Nmax, r.precision) for r in reqs},
+                            key=lambda k: -k[0])
+            for slot in self.slots:
+                for key in shapes:
+                    slot.add_graph(key)
+            torch.cuda.synchronize(self.device)
+            self.graph_build_s += time.perf_counter() - tg
+            if not self._fixed_budget:
+                self.budget = int(0.8 * torch.cuda.mem_get_info(self.device)[0])
         torch.cuda.synchronize(self.device)
         threads = []
         t0 = time.perf_counter()
