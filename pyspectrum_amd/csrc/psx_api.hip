@@ -161,10 +161,6 @@ int get_fft_plan(psx_ctx* ctx, int N, const FftPlan** out) {
 }
 
 // ----------------------------------------------------------- shell plan ----
-static int64_t shell_mmax(int j, int step) {
-  const int64_t a = (int64_t)(2 * j + 1) * step;
-  return (a * a - 1) / 4;  // largest m with shell_of_m(m, step) <= j
-}
 
 int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, const ShellPlan** out) {
   auto key = std::make_tuple(N, step, ncut, nmax, kt);

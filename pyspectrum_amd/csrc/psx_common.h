@@ -102,4 +102,13 @@ __host__ __device__ __forceinline__ int shell_of_m(int64_t m, int step) {
   return (int)((q + 1) / 2);
 }
 
+// Largest m with shell_of_m(m, step) <= j: shell_of_m is monotone in m, so
+// shell j is exactly shell_mmax(j - 1) < m <= shell_mmax(j) (j >= 1; shell 0
+// is m <= shell_mmax(0)).
+__host__ __device__ __forceinline__ int64_t shell_mmax(int j, int step) {
+  if (j < 0) return -1;
+  const int64_t a = (int64_t)(2 * j + 1) * step;
+  return (a * a - 1) / 4;
+}
+
 }  // namespace psx

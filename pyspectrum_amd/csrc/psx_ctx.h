@@ -32,7 +32,10 @@ struct FftPlan {
 
 // Row pairs (complex FFT lines) per tile of the shell fields' x pass: small
 // tiles keep several workgroups per CU to hide the gather latency.
-constexpr int SHELL_X_PAIRS = 8;
+#ifndef PSX_SHELL_X_PAIRS
+#define PSX_SHELL_X_PAIRS 8
+#endif
+constexpr int SHELL_X_PAIRS = PSX_SHELL_X_PAIRS;
 
 // One pruned-pass tile: (shell slot, a, b) with meaning per pass.
 struct Tile3 {

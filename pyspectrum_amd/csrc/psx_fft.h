@@ -67,7 +67,10 @@ inline bool fft_factor(int N, FftShape* sh) {
 
 // Lines per tile: up to 16 (FP32) / 8 (FP64) lines, at most ELEMS complex values.
 template <typename T> struct TileCfg;
-template <> struct TileCfg<float> { static constexpr int ELEMS = 8192; static constexpr int LINES = 16; };
+#ifndef PSX_TILE_LINES_F32
+#define PSX_TILE_LINES_F32 16
+#endif
+template <> struct TileCfg<float> { static constexpr int ELEMS = 8192; static constexpr int LINES = PSX_TILE_LINES_F32; };
 template <> struct TileCfg<double> { static constexpr int ELEMS = 4096; static constexpr int LINES = 8; };
 
 __host__ __device__ constexpr int tile_lines_for(int N, int elems, int lines) {

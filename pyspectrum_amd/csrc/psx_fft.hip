@@ -160,6 +160,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
   const int s = t.s, ky = t.a, kx0 = t.b;
   const int j = sd.meta[4 * s], kxw = sd.meta[4 * s + 2];
   const int kys = signed_f(ky, N);
+  // shell j is the |k|^2 range (mlo, mhi]
+  const int64_t mlo = shell_mmax(j - 1, sd.step), mhi = shell_mmax(j, sd.step);
   constexpr int NL = TileCfg<T>::LINES;  // kx columns per tile (the plan's kt)
   batched_fill<8, cplx<T>>(
       NL * N,
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
         if (kx < Nh) {
           const int kzs = signed_f(n, N);
           const int64_t m = (int64_t)kx * kx + (int64_t)kys * kys + (int64_t)kzs * kzs;
-          if (shell_of_m(m, sd.step) == j) v = H[((int64_t)n * N + ky) * Nh + kx];
+          if (m > mlo && m <= mhi) v = H[((int64_t)n * N + ky) * Nh + kx];
         }
         return v;
       },
