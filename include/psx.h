@@ -56,14 +56,17 @@ int psx_ctx_synchronize(psx_ctx* ctx);
 int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes);
 
 /* Replaces estimator.f:284-512 assign_quad (ia=ib=ic=id=0 branch), called at
- * pyspectrum.py:951.  PCS assignment with interlacing of np particles.
+ * pyspectrum.py:951 (offset 0) and :819 (offset 0.5*Ngrid).  PCS assignment
+ * with interlacing of np particles at grid coordinate kf_ks*x + 1 + offset.
  * xyz: float32 [3][np] (x block, y block, z block); w: float32 [np] or NULL
- * for unit weights; positions are clipped to [0, clip_max] (pyspectrum.py:
- * 939-941).  grid_out: complex [N^3], re = grid A (the reference's odd
- * slots), im = grid B (even slots).  wsum_abs = sum |w| sets the internal
- * fixed-point scale (order-independent accumulation). */
+ * for unit weights; positions are first clipped to [clip_lo, clip_hi]
+ * (FFT_periodic clips to [0, Lbox(1-1e-6)], pyspectrum.py:939-941), which
+ * must map inside the grid.  grid_out: complex [N^3], re = grid A (the
+ * reference's odd slots), im = grid B (even slots).  wsum_abs = sum |w| sets
+ * the internal fixed-point scale (order-independent accumulation). */
 int psx_assign_quad(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
-                    float kf_ks, float clip_max, double wsum_abs, void* grid_out, int prec);
+                    float kf_ks, float offset, float clip_lo, float clip_hi, double wsum_abs,
+                    void* grid_out, int prec);
 
 /* Replaces the transform of _FFT (pyspectrum.py:1060-1080, pyfftw ifftn
  * with normalise_idft=False): unnormalised 3D c2c FFT in place, exponent
@@ -80,6 +83,14 @@ int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid
 /* Replaces FFT_periodic (pyspectrum.py:909-959): assign + FFT + fcomb. */
 int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                      double lbox, double sumw, double wsum_abs, void* half_out, int prec);
+
+/* Replaces the grid part of FFT_survey_mono (pyspectrum.py:815-820):
+ * assign_quad with offset 0.5*Ngrid and kf_ks = float32(Ngrid/Lbox) (no
+ * clipping of positions inside (-Lbox/2, Lbox/2)), the backward FFT and
+ * estimator.f:677-745 fcomb_survey (fcomb_periodic with cf = 1/(6^3*4)), then
+ * the half-field slice.  w: the float32 weights including the FKP factor. */
+int psx_fft_survey_mono(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                        double lbox, double wsum_abs, void* half_out, int prec);
 
 /* Replaces reflect_delta (pyspectrum.py:1134-1157): half field -> full
  * field written C-order [x][y][z] (z fastest), as the reference returns it. */

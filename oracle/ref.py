@@ -64,6 +64,13 @@ def fcomb_periodic(F, sumw, Ngrid):
     return dcl
 
 
+def fcomb_survey(F, Ngrid):
+    """estimator.f:677 fcomb_survey(dcl, Ngrid) on a complex64 F-order copy."""
+    dcl = np.asfortranarray(np.asarray(F, dtype=np.complex64)).copy(order="F")
+    lib().fcomb_survey_(_p(dcl), _i(Ngrid))
+    return dcl
+
+
 def pk_pbox_rsd(dtl, irsd, Lbox, Nbin, Nmu, Ngrid):
     """estimator.f:155 pk_pbox_rsd(dtl,k,p0,p2,p4,nk,km,mk,pkm,nkm,irsd,Lbox,Nbin,Nmu,Ngrid)."""
     d = np.asfortranarray(np.asarray(dtl, dtype=np.complex64))

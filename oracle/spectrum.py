@@ -104,19 +104,21 @@ def clip_positions(xyz, Lbox):
     return out
 
 
-def _pcs_axis(r32, kf_ks32, Ngrid, shift):
+def _pcs_axis(r32, kf_ks32, Ngrid, shift, offset=0.0):
     """Cell indices (0-based, wrapped) and the 4 PCS weights x6 along one axis.
 
     Follows estimator.f:302-340 (grid A, shift 0) and :305-307, :342-378 (grid B,
-    shift 0.5): rx = kf_ks*x + 1 (float32, 1-based grid coordinate); the base
-    cell is int(rx) and h = rx - int(rx); weights (1-h)^3, 4+(3h-6)h^2,
-    6 - (others), h^3 go to base-1, base, base+1, base+2.  The float32
-    coordinate is replayed exactly (it selects the cells); weights are float64.
+    shift 0.5): rx = kf_ks*x + 1 + offset (float32, evaluated left to right;
+    1-based grid coordinate); the base cell is int(rx) and h = rx - int(rx);
+    weights (1-h)^3, 4+(3h-6)h^2, 6 - (others), h^3 go to base-1, base,
+    base+1, base+2.  The float32 coordinate is replayed exactly (it selects the
+    cells); weights are float64.
     """
     rx = (f32(kf_ks32) * r32).astype(f32) + f32(1.0)
+    rx = (rx + f32(offset)).astype(f32)
     if shift:
         rx = (rx + f32(0.5)).astype(f32)
-    base1 = rx.astype(np.int64)                  # 1-based, int() truncation, rx >= 1
+    base1 = rx.astype(np.int64)                  # 1-based, int() truncation, rx >= 0
     h = rx.astype(f64) - base1.astype(f64)
     c = (base1 - 1) % Ngrid                      # 0-based base cell (grid B wraps, :353)
     h2 = h * h
@@ -129,7 +131,7 @@ def _pcs_axis(r32, kf_ks32, Ngrid, shift):
     return idx, wts
 
 
-def assign_quad(xyz32, w, Ngrid, kf_ks32):
+def assign_quad(xyz32, w, Ngrid, kf_ks32, offset=0.0):
     """PCS assignment with interlacing: returns float64 grids (A, B), [ix, iy, iz].
 
     Grid A holds the odd (1-based) slots dtl(2i-1, j, k), grid B the even slots
@@ -141,9 +143,9 @@ def assign_quad(xyz32, w, Ngrid, kf_ks32):
     G = Ngrid ** 3
     grids = []
     for shift in (False, True):
-        ix, wx = _pcs_axis(xyz32[0], kf_ks32, Ngrid, shift)
-        iy, wy = _pcs_axis(xyz32[1], kf_ks32, Ngrid, shift)
-        iz, wz = _pcs_axis(xyz32[2], kf_ks32, Ngrid, shift)
+        ix, wx = _pcs_axis(xyz32[0], kf_ks32, Ngrid, shift, offset)
+        iy, wy = _pcs_axis(xyz32[1], kf_ks32, Ngrid, shift, offset)
+        iz, wz = _pcs_axis(xyz32[2], kf_ks32, Ngrid, shift, offset)
         idx, val = [], []
         for a in range(4):
             for b in range(4):
@@ -259,6 +261,125 @@ def FFT_periodic(xyz, w=None, Lbox=2600.0, Ngrid=360):
     F = fft_packed(A, B)
     F = fcomb_periodic(F, np.sum(w), Ngrid)
     return F[: Ngrid // 2 + 1]
+
+
+# ----------------------------------------------------------------------------
+# survey geometry (B0_survey, pyspectrum.py:13-282; FFT_survey_mono, :731-826)
+# ----------------------------------------------------------------------------
+C_KMS = 299792.458
+
+
+def comoving_distance(z, H0=67.6, Om0=0.31):
+    """Flat LCDM comoving distance [Mpc], adaptive quadrature per redshift
+    (the astropy FlatLambdaCDM(H0, Om0) background: no radiation)."""
+    from scipy.integrate import quad
+    z = np.atleast_1d(np.asarray(z, dtype=np.float64))
+    inv_e = lambda x: 1.0 / math.sqrt(Om0 * (1.0 + x) ** 3 + (1.0 - Om0))
+    out = np.empty_like(z)
+    cache = {}
+    for k, zz in enumerate(z):
+        v = cache.get(zz)
+        if v is None:
+            v = cache[zz] = quad(inv_e, 0.0, zz, epsabs=1e-15, epsrel=1e-13, limit=200)[0]
+        out[k] = v
+    return C_KMS / H0 * out
+
+
+def radecz_to_cartesian(radecz, H0=67.6, Om0=0.31):
+    """util.py:27-51: degrees -> radians, r = comoving distance * h [Mpc/h]."""
+    ra = np.array(radecz[0], dtype=np.float64) * (np.pi / 180.)
+    dec = np.array(radecz[1], dtype=np.float64) * (np.pi / 180.)
+    rad = comoving_distance(radecz[2], H0, Om0) * (H0 / 100.0)
+    return np.array([rad * np.cos(dec) * np.cos(ra), rad * np.cos(dec) * np.sin(ra),
+                     rad * np.sin(dec)])
+
+
+def fcomb_survey(F, Ngrid):
+    """estimator.f:677-745: the fcomb_periodic combination with
+    cf = 1/(6^3*4) (no division by the number of objects, :686)."""
+    return fcomb_periodic(F, 1.0, Ngrid)
+
+
+def FFT_survey_mono(radecz, nb, w=None, P0_fkp=1e6, Lbox=2600.0, Ngrid=360, H0=67.6, Om0=0.31):
+    """pyspectrum.py:731-826: cartesian positions, FKP weights
+    w/(1 + nb P0), the normalisation sums I12..I33, assignment with offset
+    0.5*Ngrid (no clipping), backward FFT, fcomb_survey; returns
+    (half field, Ntot, I12, I13, I22, I23, I33)."""
+    N = np.asarray(radecz).shape[1]
+    w = np.ones(N) if w is None else np.array(w, dtype=np.float64)
+    nb = np.asarray(nb, dtype=np.float64)
+    xyz = radecz_to_cartesian(radecz, H0, Om0)
+    assert np.sum(xyz.max(axis=1) >= 0.5 * Lbox) + np.sum(np.abs(xyz.min(axis=1)) >= 0.5 * Lbox) == 0
+    xyzs = xyz.astype(np.float32)
+    Ntot = np.sum(w)
+    w = w * (1. / (1. + nb * P0_fkp))
+    I12 = np.sum(w ** 2)
+    I13 = np.sum(w ** 3)
+    I22 = np.sum(nb * w ** 2)
+    I23 = np.sum(nb * w ** 3)
+    I33 = np.sum(nb ** 2 * w ** 3)
+    w32 = w.astype(np.float32).astype(np.float64)  # f2py REAL cast
+    A, B = assign_quad(xyzs, w32, Ngrid, f32(float(Ngrid) / Lbox), offset=f32(0.5 * Ngrid))
+    F = fcomb_survey(fft_packed(A, B), Ngrid)
+    return F[: Ngrid // 2 + 1], Ntot, I12, I13, I22, I23, I33
+
+
+def B0_survey_from_sums(braw_fn, p0k_raw, counts, alpha, I12, I13, I22, I23, I33, Nmax, Ncut,
+                        step, Ngrid):
+    """pyspectrum.py:246-282 on given shell powers and triangle sums:
+    p0k -> p0k/I22 - (1+alpha) I12/I22; for counts > 0,
+    b = (sum_x d_i d_j d_l / counts - (p_i+p_j+p_l) I23 - (1-alpha^2) I13) / I33,
+    q = b / (p_i p_j + p_j p_l + p_l p_i), counts/(fac Ngrid^3); zeros (and
+    no i_k entries) where counts = 0.  ``braw_fn(i, j, l)`` gives the sum."""
+    p0k = np.array(p0k_raw, dtype=np.float64)
+    p0k /= I22
+    p0k -= (1. + alpha) * I12 / I22
+    out = {k: [] for k in ("i_k1", "i_k2", "i_k3", "p0k1", "p0k2", "p0k3", "b123", "q123",
+                           "counts")}
+    for i in range(Ncut // step, Nmax + 1):
+        for j in range(Ncut // step, i + 1):
+            for l in range(max(i - j, Ncut // step), j + 1):
+                fac = 1.
+                if (j == l) and (i == j): fac = 6.
+                if (i == j) and (j != l): fac = 2.
+                if (i == l) and (l != j): fac = 2.
+                if (j == l) and (l != i): fac = 2.
+                c = counts[i - 1, j - 1, l - 1]
+                if c > 0:
+                    out["i_k1"].append(i); out["i_k2"].append(j); out["i_k3"].append(l)
+                    b = braw_fn(i, j, l) / c
+                    b -= (p0k[i - 1] + p0k[j - 1] + p0k[l - 1]) * I23 + (1. - alpha ** 2) * I13
+                    b /= I33
+                    out["p0k1"].append(p0k[i - 1]); out["p0k2"].append(p0k[j - 1])
+                    out["p0k3"].append(p0k[l - 1])
+                    out["b123"].append(b)
+                    out["q123"].append(b / (p0k[i - 1] * p0k[j - 1] + p0k[j - 1] * p0k[l - 1]
+                                            + p0k[l - 1] * p0k[i - 1]))
+                    out["counts"].append(c / (fac * float(Ngrid ** 3)))
+                else:
+                    for k in ("p0k1", "p0k2", "p0k3", "b123", "q123", "counts"):
+                        out[k].append(0.)
+    res = {k: np.array(v, dtype=np.float64) for k, v in out.items()}
+    for k in ("i_k1", "i_k2", "i_k3"):
+        res[k] = np.array(out[k], dtype=np.int64) * step
+    return res
+
+
+def B0_survey(radecz, nbar, w, radecz_r, nbar_r, w_r, counts, P0_fkp=1e6, Lbox=2600.0, Ngrid=360,
+              step=3, Ncut=3, Nmax=40, H0=67.6, Om0=0.31):
+    """pyspectrum.py:13-132 + 135-282: delta = delta_d - alpha delta_r with
+    alpha = Ntot_d/Ntot_r and the random catalogue's I's times alpha; shells
+    and triangle sums as in the periodic path.  ``counts``: the N_tri cache."""
+    d, Nd, *_ = FFT_survey_mono(radecz, nbar, w, P0_fkp, Lbox, Ngrid, H0, Om0)
+    r, Nr, I12r, I13r, I22r, I23r, I33r = FFT_survey_mono(radecz_r, nbar_r, w_r, P0_fkp, Lbox,
+                                                            Ngrid, H0, Om0)
+    alpha = Nd / Nr
+    I12, I13, I22, I23, I33 = (alpha * v for v in (I12r, I13r, I22r, I23r, I33r))
+    delta = reflect_delta(d, Ngrid) - alpha * reflect_delta(r, Ngrid)
+    D, p0k = shell_fields(delta, step, Ncut, Nmax)
+    fn = lambda i, j, l: np.einsum("i,i,i", D[i].ravel(), D[j].ravel(), D[l].ravel())
+    return B0_survey_from_sums(fn, p0k, counts, alpha, I12, I13, I22, I23, I33, Nmax, Ncut, step,
+                               Ngrid)
 
 
 def reflect_delta(delt, Ngrid):

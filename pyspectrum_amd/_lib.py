@@ -29,10 +29,11 @@ SIGNATURES = {
     "psx_ctx_set_stream": [_p, _p],
     "psx_ctx_synchronize": [_p],
     "psx_ctx_workspace_bytes": [_p, _c.POINTER(_i64)],
-    "psx_assign_quad": [_p, _p, _p, _i64, _i, _f, _f, _d, _p, _i],
+    "psx_assign_quad": [_p, _p, _p, _i64, _i, _f, _f, _f, _f, _d, _p, _i],
     "psx_fft3d": [_p, _p, _i, _i, _i],
     "psx_fcomb_periodic": [_p, _p, _p, _i, _d, _i],
     "psx_fft_periodic": [_p, _p, _p, _i64, _i, _d, _d, _d, _p, _i],
+    "psx_fft_survey_mono": [_p, _p, _p, _i64, _i, _d, _d, _p, _i],
     "psx_reflect_delta": [_p, _p, _p, _i, _i],
     "psx_hermitian_half": [_p, _p, _i, _i64, _i64, _i64, _p, _i, _i],
     "psx_bin_power": [_p, _p, _i, _i, _i, _d, _p, _i],

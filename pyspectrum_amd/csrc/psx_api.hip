@@ -429,22 +429,33 @@ static int check_bk_args(int N, int step, int ncut, int nmax) {
   return PSX_OK;
 }
 
-static int fft_periodic_impl(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
-                             double lbox, double sumw, double wsum_abs, void* half, int prec) {
+// Assignment, backward FFT and fcomb into the half field (FFT_periodic and the
+// grid part of FFT_survey_mono share it; fcomb_survey is fcomb_periodic with
+// cf = 1/(6^3*4), i.e. sumw = 1).
+static int mesh_to_half(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
+                        PcsMap pm, double sumw, double wsum_abs, void* half, int prec) {
   const int fp = fft_prec(prec);
   const int64_t G = (int64_t)N * N * N;
   void* grid;
   PSX_TRY(ctx_buffer(ctx, "fft_grid", G * cbytes(fp), &grid));
-  const float kf_ks = (float)((double)N / lbox);           // np.float32(float(Ngrid)/Lbox)
-  const float clip_max = (float)(lbox * (1.0 - 1e-6));     // np.clip bound, cast to float32
   const double scale = ldexp(1.0, fixed_point_exponent(wsum_abs));
   // assignment with the x pass of the backward FFT fused, then the y and z passes
-  PSX_TRY(launch_assign_grid(ctx, xyz, w, np, N, kf_ks, clip_max, scale, grid, fp, true));
+  PSX_TRY(launch_assign_grid(ctx, xyz, w, np, N, pm, scale, grid, fp, true));
   PSX_TRY(launch_fft3d_axes(ctx, grid, N, 1, fp, 1));
   const FftPlan* ft;
   PSX_TRY(get_fft_plan(ctx, N, &ft));
   PSX_TRY(launch_fcomb(ctx, grid, half, N, sumw, fp, ft->rec, ft->Wf, ft->Wd));
   return PSX_OK;
+}
+
+static int fft_periodic_impl(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
+                             double lbox, double sumw, double wsum_abs, void* half, int prec) {
+  PcsMap pm;
+  pm.kf_ks = (float)((double)N / lbox);           // np.float32(float(Ngrid)/Lbox)
+  pm.offset = 0.0f;
+  pm.lo = 0.0f;                                   // np.clip(xyz, 0, Lbox*(1-1e-6)), float32
+  pm.hi = (float)(lbox * (1.0 - 1e-6));
+  return mesh_to_half(ctx, xyz, w, np, N, pm, sumw, wsum_abs, half, prec);
 }
 
 static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
@@ -520,14 +531,21 @@ int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes) {
 }
 
 int psx_assign_quad(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
-                    float kf_ks, float clip_max, double wsum_abs, void* grid_out, int prec) {
+                    float kf_ks, float offset, float clip_lo, float clip_hi, double wsum_abs,
+                    void* grid_out, int prec) {
   PSX_REQUIRE(ctx && grid_out && (xyz || np == 0), "null argument");
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_REQUIRE(np >= 0, "negative particle count");
+  PSX_REQUIRE(clip_lo <= clip_hi && kf_ks > 0.0f, "bad clip range or kf_ks");
   PSX_TRY(check_ngrid(ngrid));
+  // every clipped position must map to rx in [0, N + 1.5]: the cell indices
+  // then wrap into the grid
+  const double rlo = (double)kf_ks * clip_lo + 1.0 + offset, rhi = (double)kf_ks * clip_hi + 1.0 + offset;
+  PSX_REQUIRE(rlo >= 0.0 && rhi <= ngrid + 1.5, "clip range maps outside the grid (kf_ks, offset)");
   const double scale = ldexp(1.0, fixed_point_exponent(wsum_abs));
-  return launch_assign_grid(ctx, xyz, w, np, ngrid, kf_ks, clip_max, scale, grid_out,
-                            fft_prec(prec), false);
+  PcsMap pm;
+  pm.kf_ks = kf_ks; pm.offset = offset; pm.lo = clip_lo; pm.hi = clip_hi;
+  return launch_assign_grid(ctx, xyz, w, np, ngrid, pm, scale, grid_out, fft_prec(prec), false);
 }
 
 int psx_fft3d(psx_ctx* ctx, void* grid, int ngrid, int sign, int prec) {
@@ -555,6 +573,22 @@ int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np,
   PSX_REQUIRE(np >= 0 && lbox > 0, "bad particle count or box size");
   PSX_TRY(check_ngrid(ngrid));
   return fft_periodic_impl(ctx, xyz, w, np, ngrid, lbox, sumw, wsum_abs, half_out, prec);
+}
+
+int psx_fft_survey_mono(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
+                        double lbox, double wsum_abs, void* half_out, int prec) {
+  PSX_REQUIRE(ctx && half_out && (xyz || np == 0), "null argument");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(np >= 0 && lbox > 0, "bad particle count or box size");
+  PSX_TRY(check_ngrid(ngrid));
+  PcsMap pm;
+  pm.kf_ks = (float)((double)ngrid / lbox);       // np.float32(float(Ngrid)/Lbox)
+  pm.offset = (float)(0.5 * ngrid);               // 0.5*Ngrid (pyspectrum.py:819)
+  // positions are not clipped by the reference (|x| < Lbox/2 is asserted,
+  // pyspectrum.py:791); the clip only keeps invalid input inside the grid
+  pm.lo = (float)(-0.5 * lbox);
+  pm.hi = (float)(0.5 * lbox);
+  return mesh_to_half(ctx, xyz, w, np, ngrid, pm, 1.0, wsum_abs, half_out, prec);
 }
 
 int psx_reflect_delta(psx_ctx* ctx, const void* half, void* full_out, int ngrid, int prec) {

@@ -117,10 +117,15 @@ inline int set_lds(K kernel, size_t bytes) {
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec);
 // line passes along axes first_axis..2 only (0 = x, 1 = y, 2 = z)
 int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis);
+// Position -> grid map of the assignment: rx = kf_ks*x + 1 + offset (float32,
+// estimator.f:302-304) after clipping x to [lo, hi].
+struct PcsMap {
+  float kf_ks, offset, lo, hi;
+};
 // PCS + interlacing assignment into the complex grid (re = grid A, im = grid B);
 // fft_x: also transform every x line (backward, unnormalised) before the store.
 int launch_assign_grid(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
-                       float kf_ks, float clip_max, double scale, void* grid, int prec, bool fft_x);
+                       PcsMap pm, double scale, void* grid, int prec, bool fft_x);
 int fixed_point_exponent(double wsum_abs);
 int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                         int prec, void* dout);

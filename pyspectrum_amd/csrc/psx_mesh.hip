@@ -9,18 +9,21 @@ namespace psx {
 
 // ------------------------------------------------------------ assignment --
 // Per-axis PCS stencil of estimator.f:302-378, replayed with the reference's
-// float32 operation order (no contraction into FMA): rx = kf_ks*x + 1 is the
-// 1-based grid coordinate; grid A uses rx, grid B uses rx + 0.5.  Returns the
-// 0-based base cell (already wrapped) and the four weights (x6) for cells
-// base-1 .. base+2.
+// float32 operation order (no contraction into FMA): rx = kf_ks*x + 1 + offset
+// is the 1-based grid coordinate (offset = 0 for boxes, Ngrid/2 for survey
+// coordinates centred on the observer, pyspectrum.py:819); grid A uses rx,
+// grid B uses rx + 0.5.  Returns the 0-based base cell (wrapped into [0, N))
+// and the four weights (x6) for cells base-1 .. base+2.
 // (HIP's __fmul_rn/__fadd_rn are plain operators, so FMA contraction is
 // switched off explicitly: the reference multiplies and adds separately.)
-__device__ __forceinline__ float pcs_coord(float r, float kf_ks, bool shift, int N, int* c) {
+__device__ __forceinline__ float pcs_coord(float r, const PcsMap& pm, bool shift, int N, int* c) {
 #pragma clang fp contract(off)
-  float rx = kf_ks * r + 1.0f;
+  float rx = pm.kf_ks * r + 1.0f;
+  rx = rx + pm.offset;
   if (shift) rx = rx + 0.5f;
-  const int i1 = (int)rx;  // rx >= 1: truncation = floor
-  *c = (i1 - 1) % N;
+  const int i1 = (int)rx;  // rx >= 0 for positions inside the clip range: truncation = floor
+  const int c0 = i1 - 1;   // in [-1, N]
+  *c = c0 < 0 ? c0 + N : (c0 >= N ? c0 - N : c0);
   return rx - (float)i1;  // exact
 }
 
@@ -51,10 +54,10 @@ template <typename W> __device__ __forceinline__ W wmul(W a, W b);
 template <> __device__ __forceinline__ float wmul<float>(float a, float b) { return a * b; }
 template <> __device__ __forceinline__ double wmul<double>(double a, double b) { return a * b; }
 
-__device__ __forceinline__ float clip_pos(float v, float clip_max) {
-  // np.clip to [0, clip_max]; NaN is sent to 0 so no cell index leaves the grid
-  v = (v >= 0.0f) ? v : 0.0f;
-  return (v <= clip_max) ? v : clip_max;
+__device__ __forceinline__ float clip_pos(float v, const PcsMap& pm) {
+  // np.clip to [lo, hi]; NaN is sent to lo so no cell index leaves the grid
+  v = (v >= pm.lo) ? v : pm.lo;
+  return (v <= pm.hi) ? v : pm.hi;
 }
 
 // The grid is assigned in tiles of whole x lines: a tile is a ty x tz block
@@ -93,10 +96,10 @@ static AssignTiling assign_tiling(int N) {
 // exclusive scan of the counts, and a scatter of (x, y, z, w) records to
 // offset + rank.  The order inside a tile depends on the atomics, which does
 // not matter: the tile's workgroup adds integer contributions.
-__device__ __forceinline__ unsigned tile_key(float y, float z, int N, float kf_ks, AssignTiling at) {
+__device__ __forceinline__ unsigned tile_key(float y, float z, int N, const PcsMap& pm, AssignTiling at) {
   int cy, cz;
-  pcs_coord(y, kf_ks, false, N, &cy);
-  pcs_coord(z, kf_ks, false, N, &cz);
+  pcs_coord(y, pm, false, N, &cy);
+  pcs_coord(z, pm, false, N, &cz);
   return (unsigned)(cz / at.tz) * (unsigned)at.nty + (unsigned)(cy / at.ty);
 }
 
@@ -105,13 +108,12 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0u;
 }
 
-__global__ void k_tile_count(const float* __restrict__ xyz, int64_t np, int N, float kf_ks,
-                             float clip_max, AssignTiling at, unsigned* __restrict__ count,
+__global__ void k_tile_count(const float* __restrict__ xyz, int64_t np, int N, PcsMap pm,
+                             AssignTiling at, unsigned* __restrict__ count,
                              unsigned* __restrict__ key, unsigned* __restrict__ rank) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const unsigned k = tile_key(clip_pos(xyz[np + i], clip_max), clip_pos(xyz[2 * np + i], clip_max),
-                              N, kf_ks, at);
+  const unsigned k = tile_key(clip_pos(xyz[np + i], pm), clip_pos(xyz[2 * np + i], pm), N, pm, at);
   key[i] = k;
   rank[i] = atomicAdd(count + k, 1u);
 }
@@ -145,13 +147,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const unsigned* __re
 
 __global__ void k_tile_scatter(const float* __restrict__ xyz, const float* __restrict__ w,
                                const unsigned* __restrict__ key, const unsigned* __restrict__ rank,
-                               const unsigned* __restrict__ off, int64_t np, float clip_max,
+                               const unsigned* __restrict__ off, int64_t np, PcsMap pm,
                                float4* __restrict__ rec) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   rec[(int64_t)off[key[i]] + rank[i]] =
-      make_float4(clip_pos(xyz[i], clip_max), clip_pos(xyz[np + i], clip_max),
-                  clip_pos(xyz[2 * np + i], clip_max), w ? w[i] : 1.0f);
+      make_float4(clip_pos(xyz[i], pm), clip_pos(xyz[np + i], pm), clip_pos(xyz[2 * np + i], pm),
+                  w ? w[i] : 1.0f);
 }
 
 // Tiles (width tw) along one axis holding the base cells c in [t0-3, t1]
@@ -180,7 +182,7 @@ __device__ __forceinline__ int wrap1(int c, int N) {
 template <typename T, int NC, bool FFTX>
 __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
     const float4* __restrict__ rec, const unsigned* __restrict__ tile_off, int64_t np, FftShape sh,
-    AssignTiling at, float kf_ks, double scale, const cplx<T>* __restrict__ tw,
+    AssignTiling at, PcsMap pm, double scale, const cplx<T>* __restrict__ tw,
     cplx<T>* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem_raw);
@@ -243,8 +245,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
       int cy, cz;
-      const float hy = pcs_coord(p[1], kf_ks, g == 1, N, &cy);
-      const float hz = pcs_coord(p[2], kf_ks, g == 1, N, &cz);
+      const float hy = pcs_coord(p[1], pm, g == 1, N, &cy);
+      const float hz = pcs_coord(p[2], pm, g == 1, N, &cz);
       int ly[4], lz[4];
       unsigned my = 0, mz = 0;
 #pragma unroll
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
       if (my == 0 || mz == 0) continue;
       int cx;
       T wx[4], wy[4], wz[4];
-      pcs_weights(pcs_coord(p[0], kf_ks, g == 1, N, &cx), wx);
+      pcs_weights(pcs_coord(p[0], pm, g == 1, N, &cx), wx);
       pcs_weights(hy, wy);
       pcs_weights(hz, wz);
       int ix[4];
@@ -338,7 +340,7 @@ int fixed_point_exponent(double wsum_abs) {
 
 template <typename T, int NC>
 static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off, int64_t np, int N,
-                          AssignTiling at, float kf_ks, double scale, cplx<T>* grid, bool fft_x) {
+                          AssignTiling at, PcsMap pm, double scale, cplx<T>* grid, bool fft_x) {
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
   const size_t nl = (size_t)at.ty * at.tz;
@@ -350,12 +352,12 @@ static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off, 
     auto kern = k_assign_lines<T, NC, true>;
     PSX_TRY(set_lds(kern, lds));
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, rec, off, np,
-                       p->shape, at, kf_ks, scale, plan_tw<T>(p), grid);
+                       p->shape, at, pm, scale, plan_tw<T>(p), grid);
   } else {
     auto kern = k_assign_lines<T, 0, false>;
     PSX_TRY(set_lds(kern, acc_bytes));
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, off,
-                       np, p->shape, at, kf_ks, scale, plan_tw<T>(p), grid);
+                       np, p->shape, at, pm, scale, plan_tw<T>(p), grid);
   }
   PSX_CHECK_LAUNCH();
   return PSX_OK;
@@ -363,11 +365,11 @@ static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off, 
 
 #define PSX_CASE_N(n)                                                                         \
   case n:                                                                                     \
-    return assign_lines_n<T, n>(ctx, rec, off, np, N, at, kf_ks, scale, grid, fft_x);
+    return assign_lines_n<T, n>(ctx, rec, off, np, N, at, pm, scale, grid, fft_x);
 
 template <typename T>
 static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
-                         float kf_ks, float clip_max, double scale, cplx<T>* grid, bool fft_x) {
+                         PcsMap pm, double scale, cplx<T>* grid, bool fft_x) {
   PSX_REQUIRE(np < (int64_t)1 << 32, "more than 2^32 - 1 particles");
   const AssignTiling at = assign_tiling<T>(N);
   const float4* rec = nullptr;
@@ -386,31 +388,31 @@ static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t
                        count, ntiles);
     PSX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_tile_count, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz,
-                       np, N, kf_ks, clip_max, at, count, key, rank);
+                       np, N, pm, at, count, key, rank);
     PSX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream,
                        (const unsigned*)count, (int)ntiles, o);
     PSX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_tile_scatter, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
                        xyz, w, (const unsigned*)key, (const unsigned*)rank, (const unsigned*)o, np,
-                       clip_max, r);
+                       pm, r);
     PSX_CHECK_LAUNCH();
     rec = r;
     off = o;
   }
-  if (!fft_x) return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, kf_ks, scale, grid, false);
+  if (!fft_x) return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, pm, scale, grid, false);
   switch (N) {
     PSX_FFT_SIZES(PSX_CASE_N)
-    default: return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, kf_ks, scale, grid, true);
+    default: return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, pm, scale, grid, true);
   }
 }
 #undef PSX_CASE_N
 
 int launch_assign_grid(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
-                       float kf_ks, float clip_max, double scale, void* grid, int prec, bool fft_x) {
+                       PcsMap pm, double scale, void* grid, int prec, bool fft_x) {
   if (prec == PSX_PREC_F64)
-    return assign_grid_t<double>(ctx, xyz, w, np, N, kf_ks, clip_max, scale, (cplx<double>*)grid, fft_x);
-  return assign_grid_t<float>(ctx, xyz, w, np, N, kf_ks, clip_max, scale, (cplx<float>*)grid, fft_x);
+    return assign_grid_t<double>(ctx, xyz, w, np, N, pm, scale, (cplx<double>*)grid, fft_x);
+  return assign_grid_t<float>(ctx, xyz, w, np, N, pm, scale, (cplx<float>*)grid, fft_x);
 }
 
 // Unit field delta = 1 (the shell indicators of _counts_Bk123), complex double.

@@ -42,10 +42,11 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from . import dat_dir
+from . import dat_dir, util
 
 __all__ = ["FFT_periodic", "reflect_delta", "Pk_periodic", "Pk_periodic_rsd", "Bk_periodic",
-           "_Bk_periodic", "_counts_Bk123", "FFTperiodic", "Bk123_periodic", "cache_dir"]
+           "_Bk_periodic", "_counts_Bk123", "FFTperiodic", "Bk123_periodic", "cache_dir",
+           "FFT_survey_mono", "B0_survey", "_B0_survey"]
 
 
 # --------------------------------------------------------------- helpers ----
@@ -435,6 +436,174 @@ def _Bk_periodic(delta, Nmax=40, Ncut=3, step=3, fft='pyfftw', nthreads=1, silen
     return {'i_k1': ijl[valid, 0].astype(np.int64) * step, 'i_k2': ijl[valid, 1].astype(np.int64) * step,
             'i_k3': ijl[valid, 2].astype(np.int64) * step, 'p0k1': p1, 'p0k2': p2, 'p0k3': p3,
             'b123': b123, 'q123': q123, 'counts': cnt}
+
+
+# ------------------------------------------------------- survey geometry ----
+def _survey_catalogue(radecz, nb, w, P0_fkp, Lbox, cosmo):
+    """Host part of FFT_survey_mono (pyspectrum.py:767-812), float64: cartesian
+    positions, the 'box not big enough' check, Ntot = sum w (before FKP), FKP
+    weights w/(1 + nb P0) and the normalisation sums I12, I13, I22, I23, I33.
+    The caller's arrays are not modified (the reference scales w in place)."""
+    if isinstance(radecz, torch.Tensor):
+        radecz = radecz.detach().cpu().numpy()
+    radecz = np.asarray(radecz)
+    N = radecz.shape[1]
+    nb = np.asarray(nb.detach().cpu().numpy() if isinstance(nb, torch.Tensor) else nb,
+                    dtype=np.float64)
+    if w is None:
+        w = np.ones(N)
+    else:
+        w = np.array(w.detach().cpu().numpy() if isinstance(w, torch.Tensor) else w,
+                     dtype=np.float64)
+    xyz = util.radecz_to_cartesian(radecz, cosmo=cosmo)
+    xyz_max = np.max(xyz, axis=1)
+    xyz_min = np.min(xyz, axis=1)
+    assert np.sum(xyz_max >= 0.5 * Lbox) + np.sum(np.abs(xyz_min) >= 0.5 * Lbox) == 0, \
+        'box not big enough!'
+    xyzs = np.ascontiguousarray(xyz.astype(np.float32))
+    Ntot = np.sum(w)
+    w = w * (1. / (1. + nb * P0_fkp))
+    I12 = np.sum(w ** 2)
+    I13 = np.sum(w ** 3)
+    I22 = np.sum(nb * w ** 2)
+    I23 = np.sum(nb * w ** 3)
+    I33 = np.sum(nb ** 2 * w ** 3)
+    assert np.all([(I12 >= 0), (I13 >= 0), (I22 >= 0), (I23 >= 0), (I33 >= 0)])
+    return xyzs, w, Ntot, (I12, I13, I22, I23, I33)
+
+
+def _survey_half_on_device(radecz, nb, w, P0_fkp, Lbox, Ngrid, cosmo, prec, device):
+    xyzs, wf, Ntot, I = _survey_catalogue(radecz, nb, w, P0_fkp, Lbox, cosmo)
+    dev = _device(device)
+    n = xyzs.shape[1]
+    x = torch.from_numpy(xyzs).to(dev)
+    w32 = wf.astype(np.float32)
+    wd = torch.from_numpy(w32).to(dev)
+    half = torch.empty((Ngrid, Ngrid, Ngrid // 2 + 1), dtype=_cdtype(prec), device=dev)
+    L.call("psx_fft_survey_mono", L.context(device).handle, L.ptr(x), L.ptr(wd), n, Ngrid,
+           float(Lbox), float(np.sum(np.abs(w32.astype(np.float64)))), L.ptr(half), prec)
+    return half, Ntot, I
+
+
+def FFT_survey_mono(radecz, nb, w=None, P0_fkp=1e6, Lbox=2600., Ngrid=360, cosmo=None, fft='pyfftw',
+                    silent=True, precision='fp32', device=0):
+    """Half field delta_0(k) of a survey catalogue (pyspectrum.py:731-826).
+
+    (RA, Dec, z) -> comoving Mpc/h (``cosmo``: an object with
+    ``comoving_distance(z)`` [Mpc] and ``h``; default FlatLambdaCDM(67.6,
+    0.31)), FKP weights w/(1 + nb P0_fkp), assignment at offset Ngrid/2,
+    backward FFT and fcomb_survey.  Returns (delta0, Ntot, I12, I13, I22,
+    I23, I33) with delta0 of shape (Ngrid//2+1, Ngrid, Ngrid) [ix, iy, iz].
+    """
+    _check_ngrid(Ngrid)
+    prec = _prec(precision)
+    half, Ntot, I = _survey_half_on_device(radecz, nb, w, P0_fkp, Lbox, Ngrid, cosmo, prec, device)
+    if not silent:
+        print('Ntot=%.2f' % Ntot)
+        for name, v in zip(('I12', 'I13', 'I22', 'I23', 'I33'), I):
+            print('%s=%.2e' % (name, v))
+    return (_as_numpy_half(half), Ntot) + tuple(I)
+
+
+def _B0_from_half(half, alpha, I12, I13, I22, I23, I33, Nmax, Ncut, step, fft, silent, prec,
+                  device):
+    """Shell fields + triangle sums on the device (psx_bk_raw), then the
+    survey normalisation and shot noise of pyspectrum.py:246-282 (float64)."""
+    Ngrid = half.shape[0]
+    dev = half.device
+    ijl = L.triangles(step, Ncut, Nmax)
+    ntri = len(ijl)
+    braw_d = torch.empty(max(ntri, 1), dtype=torch.float64, device=dev)
+    p0k_d = torch.empty(Nmax, dtype=torch.float64, device=dev)
+    L.call("psx_bk_raw", L.context(device).handle, L.ptr(half), Ngrid, step, Ncut, Nmax,
+           L.ptr(braw_d), L.ptr(p0k_d), prec)
+    counts = _counts_Bk123(Ngrid=Ngrid, Nmax=Nmax, Ncut=Ncut, step=step, fft=fft, silent=silent,
+                           device=device)
+    tc = _tri_counts(counts, ijl, Ngrid)
+    braw = braw_d.cpu().numpy()[:ntri]
+    p0k = p0k_d.cpu().numpy()
+    p0k /= I22
+    p0k -= (1. + alpha) * I12 / I22
+    G = float(Ngrid ** 3)
+    i, j, l = ijl[:, 0] - 1, ijl[:, 1] - 1, ijl[:, 2] - 1
+    fac = np.ones(ntri)
+    fac[(ijl[:, 0] == ijl[:, 1]) | (ijl[:, 1] == ijl[:, 2])] = 2.0
+    fac[(ijl[:, 0] == ijl[:, 1]) & (ijl[:, 1] == ijl[:, 2])] = 6.0
+    valid = tc > 0
+    z = np.zeros(ntri)
+    p1 = np.where(valid, p0k[i], z)
+    p2 = np.where(valid, p0k[j], z)
+    p3 = np.where(valid, p0k[l], z)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        b = braw / np.where(valid, tc, 1.0)
+        b -= (p1 + p2 + p3) * I23 + (1. - alpha ** 2) * I13
+        b /= I33
+        b123 = np.where(valid, b, z)
+        q123 = np.where(valid, b123 / (p1 * p2 + p2 * p3 + p3 * p1), z)
+    cnt = np.where(valid, tc / (fac * G), z)
+    return {'i_k1': ijl[valid, 0].astype(np.int64) * step, 'i_k2': ijl[valid, 1].astype(np.int64) * step,
+            'i_k3': ijl[valid, 2].astype(np.int64) * step, 'p0k1': p1, 'p0k2': p2, 'p0k3': p3,
+            'b123': b123, 'q123': q123, 'counts': cnt}
+
+
+def _B0_survey(delta, alpha, I12, I13, I22, I23, I33, Nmax=40, Ncut=3, step=3, fft='pyfftw',
+               nthreads=1, silent=True, precision='fp32', device=0):
+    """Survey bispectrum monopole of a full delta(k) = delta_data - alpha
+    delta_random (pyspectrum.py:135-282).  ``delta``: (Ngrid, Ngrid, Ngrid)
+    complex [ix, iy, iz]; the I's are the random catalogue's sums times alpha."""
+    d = delta if isinstance(delta, torch.Tensor) else torch.from_numpy(np.asarray(delta))
+    Ngrid = d.shape[0]
+    _check_ngrid(Ngrid)
+    prec = _prec(precision)
+    dev = _device(device)
+    in_prec = L.PREC["fp64"] if d.dtype == torch.complex128 else L.PREC["fp32"]
+    full = d.to(device=dev, dtype=_cdtype(in_prec))
+    sx, sy, sz = full.stride()
+    half = torch.empty((Ngrid, Ngrid, Ngrid // 2 + 1), dtype=_cdtype(prec), device=dev)
+    L.call("psx_hermitian_half", L.context(device).handle, L.ptr(full), in_prec, sx, sy, sz,
+           L.ptr(half), Ngrid, prec)
+    return _B0_from_half(half, alpha, I12, I13, I22, I23, I33, Nmax, Ncut, step, fft, silent, prec,
+                         device)
+
+
+def B0_survey(radecz, nbar, w=None, radecz_r=None, nbar_r=None, w_r=None, P0_fkp=1e6, Lbox=2600,
+              Ngrid=360, step=3, Ncut=3, Nmax=40, cosmo=None, fft='pyfftw', nthreads=1, silent=True,
+              precision='fp32', device=0):
+    """Bispectrum monopole in survey geometry, Scoccimarro (2015) estimator
+    (pyspectrum.py:13-132).
+
+    Data and random catalogues (RA, Dec, z) with number densities ``nbar``,
+    ``nbar_r`` and optional weights; delta = delta_data - alpha delta_random
+    with alpha = sum w / sum w_r.  Keys: i_k1, i_k2, i_k3, p0k1, p0k2, p0k3,
+    b123, q123, counts, meta.
+    """
+    if radecz_r is not None and nbar_r is None:
+        raise ValueError('specify nbar for random catalog')
+    if radecz_r is None:
+        raise ValueError('a random catalog (radecz_r, nbar_r) is required')
+    if cosmo is None:
+        cosmo = util.default_cosmology()
+    _check_ngrid(Ngrid)
+    prec = _prec(precision)
+    kf = 2 * np.pi / Lbox
+    N = np.shape(radecz)[1]
+    half_d, Ngtot, _ = _survey_half_on_device(radecz, nbar, w, P0_fkp, Lbox, Ngrid, cosmo, prec,
+                                              device)
+    half_r, Nrtot, Ir = _survey_half_on_device(radecz_r, nbar_r, w_r, P0_fkp, Lbox, Ngrid, cosmo,
+                                               prec, device)
+    alpha = Ngtot / Nrtot
+    if not silent:
+        print('alpha=%e' % alpha)
+    I12, I13, I22, I23, I33 = (alpha * v for v in Ir)
+    # delta_d - alpha delta_r on the half fields (reflect_delta is linear); the
+    # product and the difference are rounded separately, as in numpy
+    half = half_d - alpha * half_r
+    del half_d, half_r
+    bispec = _B0_from_half(half, alpha, I12, I13, I22, I23, I33, Nmax, Ncut, step, fft, True, prec,
+                           device)
+    bispec['meta'] = {'Lbox': Lbox, 'Ngrid': Ngrid, 'step': step, 'Ncut': Ncut, 'Nmax': Nmax, 'N': N,
+                      'nbar': nbar, 'kf': kf}
+    return bispec
 
 
 # names used by older drivers of the reference (run/_test_*.py)

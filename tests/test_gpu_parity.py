@@ -75,7 +75,7 @@ def test_assign_quad_matches_oracle(prec):
     x = torch.from_numpy(np.ascontiguousarray(xyz.astype(np.float32))).to(DEV)
     wd = torch.from_numpy(w.astype(np.float32)).to(DEV)
     g = torch.empty((N, N, N), dtype=_cdt(prec), device=DEV)
-    L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N, float(kf_ks),
+    L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N, float(kf_ks), 0.0, 0.0,
            float(np.float32(Lbox * (1 - 1e-6))), float(np.abs(w).sum()), L.ptr(g), L.PREC[prec])
     out = g.cpu().numpy().transpose(2, 1, 0)  # -> [ix, iy, iz]
     tol = 1e-11 if prec == "fp64" else 5e-7  # fp32: float32 stencil products
@@ -106,7 +106,7 @@ def test_assign_edge_cases(prec, case):
     x = torch.from_numpy(np.ascontiguousarray(xyz.astype(np.float32))).to(DEV)
     wd = torch.from_numpy(w.astype(np.float32)).to(DEV)
     g = torch.empty((N, N, N), dtype=_cdt(prec), device=DEV)
-    L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N, float(kf_ks),
+    L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N, float(kf_ks), 0.0, 0.0,
            float(np.float32(Lbox * (1 - 1e-6))), float(np.abs(w).sum()), L.ptr(g), L.PREC[prec])
     out = g.cpu().numpy().transpose(2, 1, 0)
     tol = 1e-11 if prec == "fp64" else 5e-7
@@ -124,7 +124,8 @@ def test_assign_is_deterministic():
     for _ in range(2):
         g = torch.empty((N, N, N), dtype=torch.complex64, device=DEV)
         L.call("psx_assign_quad", _h(), L.ptr(x), None, x.shape[1], N, float(np.float32(N / Lbox)),
-               float(np.float32(Lbox * (1 - 1e-6))), float(x.shape[1]), L.ptr(g), L.PREC["fp32"])
+               0.0, 0.0, float(np.float32(Lbox * (1 - 1e-6))), float(x.shape[1]), L.ptr(g),
+               L.PREC["fp32"])
         outs.append(g.cpu().numpy())
     assert np.array_equal(outs[0], outs[1])
 
