@@ -44,11 +44,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-triangles", type=int, default=12)
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--config", default="box", choices=["box", "serve", "batch", "pmc"],
+    ap.add_argument("--config", default="box", choices=["box", "serve", "batch", "pmc", "survey"],
                     help="box: the single clustered box (default); serve: Poisson request "
                          "stream at --rate/s; batch: 64 uneven boxes submitted at once; "
                          "pmc: one box, a 1 GiB device copy (counter calibration), one box "
-                         "-- the program to run under rocprofv3 --pmc")
+                         "-- the program to run under rocprofv3 --pmc; survey: B0_survey on a "
+                         "synthetic footprint (--npart galaxies, 4x randoms)")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--nreq", type=int, default=40)
     ap.add_argument("--slots", type=int, default=4)
@@ -253,6 +254,61 @@ def run_stream(args):
     print(json.dumps(line))
 
 
+def run_survey(args):
+    """Survey-geometry B(k) (B0_survey): a synthetic footprint (RA 0-90 deg,
+    Dec -30..30 deg, z 0.2-0.6, n(z) smooth) of --npart galaxies and 4x as many
+    randoms at Ngrid=360, Lbox=3600 Mpc/h.  Times the whole call (catalogues
+    copied to the device, positions/weights/sums by psx_survey_catalogue,
+    assignment, FFT, shells, contraction) for the split-BF16 and FP64
+    pipelines, and for comparison the reference-style numpy catalogue
+    preparation of both catalogues on the host."""
+    import torch
+    from pyspectrum_amd import pyspectrum as P
+    from pyspectrum_amd import util
+    rng = np.random.default_rng(args.seed)
+    cosmo = util.default_cosmology()
+
+    def catalogue(n):
+        u = rng.uniform(size=n)
+        dec = np.degrees(np.arcsin(np.sin(np.radians(-30.)) + u * 2 * np.sin(np.radians(30.))))
+        radecz = np.array([rng.uniform(0, 90, n), dec, rng.uniform(0.2, 0.6, n)])
+        nb = 3e-4 * np.exp(-((radecz[2] - 0.4) / 0.15) ** 2)
+        return radecz, nb
+
+    nd = args.npart
+    d, nbd = catalogue(nd)
+    r, nbr = catalogue(4 * nd)
+    kw = dict(P0_fkp=1e4, Lbox=3600., Ngrid=360, step=args.step, Ncut=args.ncut, Nmax=args.nmax,
+              cosmo=cosmo)
+    t0 = time.perf_counter()
+    P._survey_catalogue(d, nbd, None, 1e4, 3600., cosmo)
+    P._survey_catalogue(r, nbr, None, 1e4, 3600., cosmo)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    res, times = {}, {}
+    for prec in (args.precision, "fp64"):
+        P.B0_survey(d, nbd, radecz_r=r, nbar_r=nbr, precision=prec, **kw)  # warm-up (plans, counts)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            res[prec] = P.B0_survey(d, nbd, radecz_r=r, nbar_r=nbr, precision=prec, **kw)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        times[prec] = float(np.median(ts))
+    a, b = res[args.precision], res["fp64"]
+    big = np.max(np.abs(b["b123"]))
+    err = float(np.max(np.abs(a["b123"] - b["b123"])) / big)
+    print(json.dumps({
+        "metric": "survey B(k) catalogues/s (B0_survey, data + 4x randoms)",
+        "value": round(1e3 / times[args.precision], 3), "unit": "catalogues/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": args.precision, "data": "synthetic survey footprint",
+        "config": {"workload": "%d galaxies + %d randoms, RA 0-90, Dec -30..30, z 0.2-0.6, "
+                               "Ngrid=360, Lbox=3600, step=%d, Ncut=%d, Nmax=%d"
+                               % (nd, 4 * nd, args.step, args.ncut, args.nmax)},
+        "ms_per_call": round(times[args.precision], 2), "fp64_ms_per_call": round(times["fp64"], 2),
+        "host_numpy_catalogue_ms": round(host_ms, 2),
+        "b123_err_vs_fp64_rel_to_max": err, "triangles": int(len(a["b123"]))}))
+
+
 def run_pmc(args):
     """Short program for hardware-counter passes: the box once (plans, counts
     and buffers warm), a 1 GiB device-to-device copy whose FETCH_SIZE and
@@ -276,6 +332,8 @@ def main():
     args = parse()
     if args.gpus == 1 and args.config == "pmc":
         return run_pmc(args)
+    if args.gpus == 1 and args.config == "survey":
+        return run_survey(args)
     if args.gpus == 1 and args.config != "box":
         return run_stream(args)
     if args.gpus != 1:

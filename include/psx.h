@@ -84,6 +84,22 @@ int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid
 int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                      double lbox, double sumw, double wsum_abs, void* half_out, int prec);
 
+/* Replaces the per-object work of FFT_survey_mono (pyspectrum.py:767-812)
+ * with radecz_to_cartesian (util.py:27-51), in float64 on the device:
+ * radecz: device float64 [3][n] (RA, Dec in degrees, redshift in
+ * [0, zmax]); nb: device float64 [n]; w: device float64 [n] or NULL (unit
+ * weights).  The comoving distance is the cubic Hermite interpolant of a
+ * host table: dist_nodes[k] = D(k zmax/nseg) [Mpc] and dist_slopes[k] =
+ * (zmax/nseg) dD/dz there (k = 0..nseg).  Writes xyz_out (float32 [3][n],
+ * Mpc/h = D h) and w_out (float32 [n], w/(1 + nb p0_fkp)) and, on the host,
+ * sums_out[PSX_SURVEY_SUMS] = Ntot, I12, I13, I22, I23, I33, sum|w_out|,
+ * max x, max y, max z, max -x, max -y, max -z.  Synchronises the stream. */
+#define PSX_SURVEY_SUMS 13
+int psx_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, const double* w,
+                         int64_t n, const double* dist_nodes, const double* dist_slopes, int nseg,
+                         double zmax, double h, double p0_fkp, float* xyz_out, float* w_out,
+                         double* sums_out);
+
 /* Replaces the grid part of FFT_survey_mono (pyspectrum.py:815-820):
  * assign_quad with offset 0.5*Ngrid and kf_ks = float32(Ngrid/Lbox) (no
  * clipping of positions inside (-Lbox/2, Lbox/2)), the backward FFT and

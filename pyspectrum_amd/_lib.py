@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libpsx.so")
 
 PSX_OK = 0
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+SURVEY_SUMS = 13  # PSX_SURVEY_SUMS
 
 _c = ctypes
 _p = _c.c_void_p
@@ -34,6 +35,7 @@ SIGNATURES = {
     "psx_fcomb_periodic": [_p, _p, _p, _i, _d, _i],
     "psx_fft_periodic": [_p, _p, _p, _i64, _i, _d, _d, _d, _p, _i],
     "psx_fft_survey_mono": [_p, _p, _p, _i64, _i, _d, _d, _p, _i],
+    "psx_survey_catalogue": [_p, _p, _p, _p, _i64, _p, _p, _i, _d, _d, _d, _p, _p, _p],
     "psx_reflect_delta": [_p, _p, _p, _i, _i],
     "psx_hermitian_half": [_p, _p, _i, _i64, _i64, _i64, _p, _i, _i],
     "psx_bin_power": [_p, _p, _i, _i, _i, _d, _p, _i],

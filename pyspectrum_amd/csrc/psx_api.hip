@@ -575,6 +575,17 @@ int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np,
   return fft_periodic_impl(ctx, xyz, w, np, ngrid, lbox, sumw, wsum_abs, half_out, prec);
 }
 
+int psx_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, const double* w,
+                         int64_t n, const double* dist_nodes, const double* dist_slopes, int nseg,
+                         double zmax, double h, double p0_fkp, float* xyz_out, float* w_out,
+                         double* sums_out) {
+  PSX_REQUIRE(ctx && sums_out && dist_nodes && dist_slopes, "null argument");
+  PSX_REQUIRE(n == 0 || (radecz && nb && xyz_out && w_out), "null argument");
+  PSX_REQUIRE(n >= 0 && nseg >= 1 && zmax >= 0.0, "bad object count, table size or zmax");
+  return launch_survey_catalogue(ctx, radecz, nb, w, n, dist_nodes, dist_slopes, nseg, zmax, h,
+                                 p0_fkp, xyz_out, w_out, sums_out);
+}
+
 int psx_fft_survey_mono(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                         double lbox, double wsum_abs, void* half_out, int prec) {
   PSX_REQUIRE(ctx && half_out && (xyz || np == 0), "null argument");

@@ -130,4 +130,8 @@ int fixed_point_exponent(double wsum_abs);
 int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                         int prec, void* dout);
 int launch_fill_unit(psx_ctx* ctx, double* half, int64_t n);
+int launch_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, const double* w,
+                            int64_t n, const double* dist_nodes, const double* dist_slopes, int nseg,
+                            double zmax, double h, double p0, float* xyz, float* wout,
+                            double* sums);
 }  // namespace psx

@@ -473,16 +473,49 @@ def _survey_catalogue(radecz, nb, w, P0_fkp, Lbox, cosmo):
 
 
 def _survey_half_on_device(radecz, nb, w, P0_fkp, Lbox, Ngrid, cosmo, prec, device):
-    xyzs, wf, Ntot, I = _survey_catalogue(radecz, nb, w, P0_fkp, Lbox, cosmo)
+    """FFT_survey_mono on the device: psx_survey_catalogue (positions, FKP
+    weights, normalisation sums), the box check, then psx_fft_survey_mono."""
+    if cosmo is None:
+        cosmo = util.default_cosmology()
     dev = _device(device)
-    n = xyzs.shape[1]
-    x = torch.from_numpy(xyzs).to(dev)
-    w32 = wf.astype(np.float32)
-    wd = torch.from_numpy(w32).to(dev)
+
+    def dev64(a, shape):
+        t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.asarray(a, dtype=np.float64))
+        t = t.to(device=dev, dtype=torch.float64).contiguous()
+        if tuple(t.shape) != shape:
+            raise ValueError("expected shape %s, got %s" % (shape, tuple(t.shape)))
+        return t
+
+    n = radecz.shape[1]
+    rd = dev64(radecz, (3, n))
+    nbd = dev64(nb, (n,))
+    wd_in = None if w is None else dev64(w, (n,))
+    z = rd[2]
+    zmax = float(z.max().item()) if n else 0.0
+    if n and (not bool(torch.isfinite(z).all().item()) or float(z.min().item()) < 0):
+        raise ValueError("redshifts must be finite and non-negative")
+    Dn, sl = util.distance_table(cosmo, zmax)
+    xyz = torch.empty((3, n), dtype=torch.float32, device=dev)
+    wf = torch.empty(n, dtype=torch.float32, device=dev)
+    sums = np.zeros(L.SURVEY_SUMS)
+    ctx = L.context(device)
+    L.call("psx_survey_catalogue", ctx.handle, L.ptr(rd), L.ptr(nbd), L.ptr(wd_in), n,
+           Dn.ctypes.data_as(L._p), sl.ctypes.data_as(L._p), len(Dn) - 1, zmax,
+           float(_value_h(cosmo)), float(P0_fkp), L.ptr(xyz), L.ptr(wf),
+           sums.ctypes.data_as(L._p))
+    Ntot, I12, I13, I22, I23, I33, sabs = (float(v) for v in sums[:7])
+    xyz_max, xyz_min = sums[7:10], -sums[10:13]
+    assert np.sum(xyz_max >= 0.5 * Lbox) + np.sum(np.abs(xyz_min) >= 0.5 * Lbox) == 0, \
+        'box not big enough!'
+    assert np.all([(I12 >= 0), (I13 >= 0), (I22 >= 0), (I23 >= 0), (I33 >= 0)])
     half = torch.empty((Ngrid, Ngrid, Ngrid // 2 + 1), dtype=_cdtype(prec), device=dev)
-    L.call("psx_fft_survey_mono", L.context(device).handle, L.ptr(x), L.ptr(wd), n, Ngrid,
-           float(Lbox), float(np.sum(np.abs(w32.astype(np.float64)))), L.ptr(half), prec)
-    return half, Ntot, I
+    L.call("psx_fft_survey_mono", ctx.handle, L.ptr(xyz), L.ptr(wf), n, Ngrid, float(Lbox), sabs,
+           L.ptr(half), prec)
+    return half, Ntot, (I12, I13, I22, I23, I33)
+
+
+def _value_h(cosmo):
+    return float(np.asarray(getattr(cosmo.h, "value", cosmo.h)))
 
 
 def FFT_survey_mono(radecz, nb, w=None, P0_fkp=1e6, Lbox=2600., Ngrid=360, cosmo=None, fft='pyfftw',

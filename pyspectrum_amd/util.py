@@ -52,17 +52,35 @@ class FlatLambdaCDM:
         nodes = mid[:, None] + hw[:, None] * x[None, :]
         seg = (hw[:, None] * wq[None, :] / self.efunc(nodes)).sum(axis=1)
         F = np.concatenate([[0.0], np.cumsum(seg)])
-        k = np.clip(np.searchsorted(edges, z, side="right") - 1, 0, M - 1)
-        h = edges[k + 1] - edges[k]
-        t = (z - edges[k]) / h
-        d0, d1 = 1.0 / self.efunc(edges[k]), 1.0 / self.efunc(edges[k + 1])
-        t2, t3 = t * t, t * t * t
-        return ((2 * t3 - 3 * t2 + 1) * F[k] + (t3 - 2 * t2 + t) * h * d0
-                + (-2 * t3 + 3 * t2) * F[k + 1] + (t3 - t2) * h * d1)
+        hd = (zmax / M) / self.efunc(edges)        # interval width x derivative at the nodes
+        s = z * (M / zmax)                         # uniform grid: interval index directly
+        k = np.minimum(s.astype(np.int64), M - 1)
+        t = s - k
+        t2 = t * t
+        t3 = t2 * t
+        return ((2 * t3 - 3 * t2 + 1) * F[k] + (t3 - 2 * t2 + t) * hd[k]
+                + (-2 * t3 + 3 * t2) * F[k + 1] + (t3 - t2) * hd[k + 1])
 
     def comoving_distance(self, z):
         """Line-of-sight comoving distance in Mpc (float64 array or scalar)."""
         return (C_KMS / self.H0) * self._chi(z)
+
+
+def _value(q):
+    """Plain float64 array of a number, array or astropy Quantity."""
+    return np.asarray(getattr(q, "value", q), dtype=np.float64)
+
+
+def distance_table(cosmo, zmax, nseg=4096):
+    """Comoving distance D [Mpc] of ``cosmo`` at nseg+1 uniform nodes on
+    [0, zmax] and the node slopes times the spacing, (zmax/nseg) (c/H0)/E(z),
+    for cubic Hermite interpolation (the device catalogue kernel)."""
+    zmax = float(zmax)
+    edges = np.linspace(0.0, zmax, nseg + 1)
+    D = _value(cosmo.comoving_distance(edges))
+    H0 = float(_value(cosmo.H0))
+    slopes = (zmax / nseg) * (C_KMS / H0) / _value(cosmo.efunc(edges))
+    return np.ascontiguousarray(D), np.ascontiguousarray(slopes)
 
 
 def default_cosmology():

@@ -144,3 +144,35 @@ def test_b0_survey_full_field_entry_matches():
     for k in ("p0k1", "b123", "q123", "counts"):
         floor = 1e-6 * np.max(np.abs(b[k]))
         assert _rel(a[k], b[k], floor) < 1e-10, k
+
+
+def test_device_catalogue_matches_host():
+    # psx_survey_catalogue (float64 on the device) against the host numpy
+    # restatement of pyspectrum.py:767-812: float32 positions agree to one
+    # float32 ulp (float64 cos/sin may differ by an ulp between libraries),
+    # weights bit for bit, sums to float64 round-off of the summation order
+    _say("device survey catalogue")
+    d, nb, w = _catalogue(50000, 28, zlo=0.0, zhi=0.6)
+    c = util.default_cosmology()
+    xyz_h, w_h, Ntot_h, I_h = P._survey_catalogue(d, nb, w, 1e4, 4000.0, c)
+    dev = torch.device("cuda", 0)
+    rd = torch.from_numpy(np.ascontiguousarray(d)).to(dev)
+    nbd = torch.from_numpy(nb).to(dev)
+    wd = torch.from_numpy(w).to(dev)
+    zmax = float(d[2].max())
+    Dn, sl = util.distance_table(c, zmax)
+    xyz = torch.empty((3, d.shape[1]), dtype=torch.float32, device=dev)
+    wf = torch.empty(d.shape[1], dtype=torch.float32, device=dev)
+    sums = np.zeros(L.SURVEY_SUMS)
+    L.call("psx_survey_catalogue", L.context(0).handle, L.ptr(rd), L.ptr(nbd), L.ptr(wd), d.shape[1],
+           Dn.ctypes.data_as(L._p), sl.ctypes.data_as(L._p), len(Dn) - 1, zmax, c.h, 1e4, L.ptr(xyz),
+           L.ptr(wf), sums.ctypes.data_as(L._p))
+    xg = xyz.cpu().numpy()
+    ulp = np.spacing(np.abs(xyz_h).astype(np.float32))
+    assert np.all(np.abs(xg - xyz_h) <= ulp)
+    assert np.mean(xg != xyz_h) < 1e-3
+    assert np.array_equal(wf.cpu().numpy(), w_h.astype(np.float32))
+    exp = [Ntot_h] + list(I_h)
+    assert np.allclose(sums[:6], exp, rtol=1e-12, atol=0)
+    assert np.allclose(sums[7:10], xyz_h.astype(np.float64).max(axis=1), rtol=1e-6)
+    assert np.allclose(-sums[10:13], xyz_h.astype(np.float64).min(axis=1), rtol=1e-6)
