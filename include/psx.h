@@ -84,7 +84,7 @@ int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid
 int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                      double lbox, double sumw, double wsum_abs, void* half_out, int prec);
 
-/* Replaces the per-object work of FFT_survey_mono (pyspectrum.py:767-812)
+/* Replaces the per-object work of FFT_survey_mono (pyspectrum.py:767-806)
  * with radecz_to_cartesian (util.py:27-51), in float64 on the device:
  * radecz: device float64 [3][n] (RA, Dec in degrees, redshift in
  * [0, zmax]); nb: device float64 [n]; w: device float64 [n] or NULL (unit
@@ -100,7 +100,7 @@ int psx_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, c
                          double zmax, double h, double p0_fkp, float* xyz_out, float* w_out,
                          double* sums_out);
 
-/* Replaces the grid part of FFT_survey_mono (pyspectrum.py:815-820):
+/* Replaces the grid part of FFT_survey_mono (pyspectrum.py:817-822):
  * assign_quad with offset 0.5*Ngrid and kf_ks = float32(Ngrid/Lbox) (no
  * clipping of positions inside (-Lbox/2, Lbox/2)), the backward FFT and
  * estimator.f:677-745 fcomb_survey (fcomb_periodic with cf = 1/(6^3*4)), then

@@ -596,7 +596,7 @@ int psx_fft_survey_mono(psx_ctx* ctx, const float* xyz, const float* w, int64_t 
   pm.kf_ks = (float)((double)ngrid / lbox);       // np.float32(float(Ngrid)/Lbox)
   pm.offset = (float)(0.5 * ngrid);               // 0.5*Ngrid (pyspectrum.py:819)
   // positions are not clipped by the reference (|x| < Lbox/2 is asserted,
-  // pyspectrum.py:791); the clip only keeps invalid input inside the grid
+  // pyspectrum.py:785); the clip only keeps invalid input inside the grid
   pm.lo = (float)(-0.5 * lbox);
   pm.hi = (float)(0.5 * lbox);
   return mesh_to_half(ctx, xyz, w, np, ngrid, pm, 1.0, wsum_abs, half_out, prec);

@@ -1,5 +1,5 @@
 // Survey catalogue preparation on the device: the per-object work of
-// FFT_survey_mono (pyspectrum.py:767-812) and radecz_to_cartesian
+// FFT_survey_mono (pyspectrum.py:767-806) and radecz_to_cartesian
 // (util.py:27-51) -- comoving positions, FKP weights and the normalisation
 // sums -- in float64, positions and weights stored as float32 for the
 // assignment.

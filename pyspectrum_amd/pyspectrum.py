@@ -440,7 +440,7 @@ def _Bk_periodic(delta, Nmax=40, Ncut=3, step=3, fft='pyfftw', nthreads=1, silen
 
 # ------------------------------------------------------- survey geometry ----
 def _survey_catalogue(radecz, nb, w, P0_fkp, Lbox, cosmo):
-    """Host part of FFT_survey_mono (pyspectrum.py:767-812), float64: cartesian
+    """Host part of FFT_survey_mono (pyspectrum.py:767-806), float64: cartesian
     positions, the 'box not big enough' check, Ntot = sum w (before FKP), FKP
     weights w/(1 + nb P0) and the normalisation sums I12, I13, I22, I23, I33.
     The caller's arrays are not modified (the reference scales w in place)."""
