@@ -22,7 +22,7 @@ int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int
 int launch_contract(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax, int prec,
                     double* braw);
 int contract_maxpt(int prec, int nlt);
-int contract_waves_per_group();
+int contract_waves_per_group(int prec, int nlt);
 int launch_bk_finalize(psx_ctx* ctx, const double* braw, const double* p0k, const double* cnt,
                        const int* tri, int ntri, double G, double nbar, double a3, double b3,
                        double a6, double b6, double* out);
@@ -361,7 +361,7 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
   for (int m : ptmask) cp->nblk += __builtin_popcount(m);
   // assign pair tiles to waves: groups of WPG waves x maxpt slots, greedy by load
   const int maxpt = contract_maxpt(prec, cp->nlt);
-  const int WPG = contract_waves_per_group();
+  const int WPG = contract_waves_per_group(prec, cp->nlt);
   const int ngroups = (int)cdiv(cp->npt, WPG * maxpt);
   const int nw = ngroups * WPG;
   std::vector<int> order(cp->npt);

@@ -39,14 +39,18 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 
-// Compile-time tuning: waves per workgroup, pair tiles per wave and points
-// per chunk in the split-BF16 mode, and the minimum waves per SIMD that
-// register allocation must allow (1 = no constraint).
+// Compile-time tuning: waves per workgroup (FP64/FP32, and split-BF16 at 4 l
+// tiles), waves and pair tiles per wave of the split-BF16 mode up to 3 l
+// tiles, points per chunk in the split-BF16 mode, and the minimum waves per
+// SIMD that register allocation must allow (1 = no constraint).
 #ifndef PSX_CT_WAVES
 #define PSX_CT_WAVES 8
 #endif
+#ifndef PSX_CT_WAVES_BF
+#define PSX_CT_WAVES_BF 12
+#endif
 #ifndef PSX_CT_MAXPT_BF
-#define PSX_CT_MAXPT_BF 4
+#define PSX_CT_MAXPT_BF 2
 #endif
 #ifndef PSX_CT_MINW
 #define PSX_CT_MINW 1
@@ -54,8 +58,6 @@ typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 #ifndef PSX_CT_KC_BF
 #define PSX_CT_KC_BF 128
 #endif
-constexpr int CT_WAVES = PSX_CT_WAVES;
-constexpr int CT_THREADS = CT_WAVES * 64;
 constexpr int CT_MAXLT = 4;   // l tiles in the plan layout (ns <= 64)
 
 template <int MODE> struct CtTraits;
@@ -69,13 +71,19 @@ template <> struct CtTraits<PSX_PREC_F32> {
 };
 template <> struct CtTraits<PSX_PREC_BF16X3> {
   typedef float TD; typedef f4v Acc; typedef double TP;
-  static constexpr int KC = PSX_CT_KC_BF, MAXPT = PSX_CT_MAXPT_BF, PAD = 4;
+  static constexpr int KC = PSX_CT_KC_BF, MAXPT = 3, PAD = 4;
 };
 
-// Pair tiles per wave: the mode's MAXPT, one fewer in the split-BF16 mode at
-// 4 l tiles (its float32 and float64 accumulators must fit in 256 VGPRs).
+// Waves per workgroup and pair tiles per wave.  Split-BF16 up to 3 l tiles:
+// 12 waves x 2 tiles (152 VGPRs, so 3 waves per SIMD overlap each other's LDS,
+// VALU and MFMA latencies; measured 4.5 % faster than 8 x 4 at Nmax 40).
+// Otherwise 8 waves with the mode's tiles per wave (split-BF16 at 4 l tiles:
+// 3, whose float32 and float64 accumulators fit 256 VGPRs at 2 waves per SIMD).
+template <int MODE, int NLT> __host__ __device__ constexpr int ct_waves() {
+  return (MODE == PSX_PREC_BF16X3 && NLT <= 3) ? PSX_CT_WAVES_BF : PSX_CT_WAVES;
+}
 template <int MODE, int NLT> __host__ __device__ constexpr int ct_maxpt() {
-  return (MODE == PSX_PREC_BF16X3 && NLT > 3) ? CtTraits<MODE>::MAXPT - 1 : CtTraits<MODE>::MAXPT;
+  return (MODE == PSX_PREC_BF16X3 && NLT <= 3) ? PSX_CT_MAXPT_BF : CtTraits<MODE>::MAXPT;
 }
 
 // Row strides (elements): padded so that 8 consecutive rows start in
@@ -153,7 +161,7 @@ __device__ __forceinline__ void product_split_bf16(const f4v& a0, const f4v& a1,
 // D: [ns][G] shell fields; partials: [gridDim.y][gridDim.x][nslot][256].
 // NLT: l tiles read (ns <= 16*NLT).
 template <int MODE, int NLT>
-__global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
+__global__ __launch_bounds__((ct_waves<MODE, NLT>() * 64), PSX_CT_MINW) void k_contract(
     const typename CtTraits<MODE>::TD* __restrict__ D, int64_t G, int ns,
     const int* __restrict__ wave_pt, const int* __restrict__ wave_mask,
     const int* __restrict__ wave_slot, const int* __restrict__ pair_s, int64_t pts_per_block,
@@ -162,6 +170,7 @@ __global__ __launch_bounds__(CT_THREADS, PSX_CT_MINW) void k_contract(
   typedef typename Tr::TD TD;
   typedef typename Tr::Acc Acc;
   constexpr int KC = Tr::KC, MAXPT = ct_maxpt<MODE, NLT>();
+  constexpr int CT_WAVES = ct_waves<MODE, NLT>(), CT_THREADS = CT_WAVES * 64;
   constexpr int LDW = ct_ldw<MODE>();
   constexpr int LDWH = ct_ldwh<MODE>();
   constexpr bool BF = MODE == PSX_PREC_BF16X3;
@@ -451,7 +460,8 @@ static int contract_slots(psx_ctx* ctx, size_t lds, int* out) {
   auto it = cache.find(key);
   if (it != cache.end()) { *out = it->second; return PSX_OK; }
   int per_cu = 0, cus = 0;
-  PSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_contract<MODE, NLT>, CT_THREADS, lds));
+  PSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_contract<MODE, NLT>,
+                                                       ct_waves<MODE, NLT>() * 64, lds));
   PSX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
   if (per_cu < 1) per_cu = 1;
   *out = cache[key] = per_cu * cus;
@@ -463,6 +473,8 @@ static int contract_t(psx_ctx* ctx, const ContractPlan* cp, const void* D, int N
   typedef CtTraits<MODE> Tr;
   PSX_REQUIRE(cp->maxpt == (ct_maxpt<MODE, NLT>()), "contraction plan built for another precision");
   const int64_t G = (int64_t)N * N * N;
+  constexpr int CT_WAVES = ct_waves<MODE, NLT>();
+  PSX_REQUIRE(cp->waves % CT_WAVES == 0, "contraction plan built for another wave layout");
   const int ngroups = cp->waves / CT_WAVES;
   const size_t lds = ct_lds_bytes<MODE>(cp->ns);
   PSX_TRY(set_lds(k_contract<MODE, NLT>, lds));
@@ -477,7 +489,7 @@ static int contract_t(psx_ctx* ctx, const ContractPlan* cp, const void* D, int N
   void* part = nullptr;
   PSX_TRY(ctx_buffer(ctx, "contract_partials",
                      (size_t)ngroups * nwg * nslot * 256 * sizeof(typename Tr::TP), &part));
-  hipLaunchKernelGGL((k_contract<MODE, NLT>), dim3((unsigned)nwg, ngroups), dim3(CT_THREADS), lds,
+  hipLaunchKernelGGL((k_contract<MODE, NLT>), dim3((unsigned)nwg, ngroups), dim3(CT_WAVES * 64), lds,
                      ctx->stream, (const typename Tr::TD*)D, G, cp->ns, cp->d_wave_pt,
                      cp->d_wave_mask, cp->d_wave_slot, cp->d_pair_s, pts_per_block, nslot,
                      (typename Tr::TP*)part);
@@ -509,7 +521,11 @@ static int contract_mode(psx_ctx* ctx, const void* D, int N, int step, int ncut,
   }
 }
 
-int contract_waves_per_group() { return CT_WAVES; }
+int contract_waves_per_group(int prec, int nlt) {
+  if (prec == PSX_PREC_F64) return ct_waves<PSX_PREC_F64, 3>();
+  if (prec == PSX_PREC_F32) return ct_waves<PSX_PREC_F32, 3>();
+  return nlt > 3 ? ct_waves<PSX_PREC_BF16X3, 4>() : ct_waves<PSX_PREC_BF16X3, 3>();
+}
 
 int contract_maxpt(int prec, int nlt) {
   if (prec == PSX_PREC_F64) return CtTraits<PSX_PREC_F64>::MAXPT;

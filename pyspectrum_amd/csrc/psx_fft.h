@@ -39,6 +39,9 @@ struct FftShape {
 };
 
 // Factor N = 2^a 3^b 5^c into stages: 5s, 9s, a 3, a 4 or 2, then 8s.
+#ifndef PSX_FFT_ORDER
+#define PSX_FFT_ORDER 0
+#endif
 __host__ __device__ constexpr FftShape fft_shape_of(int N) {
   FftShape sh{N, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int n = N, c2 = 0, c3 = 0, c5 = 0;
@@ -48,12 +51,28 @@ __host__ __device__ constexpr FftShape fft_shape_of(int N) {
   if (n != 1) { sh.nstages = -1; return sh; }
   int r[FFT_MAX_STAGES + 8] = {};
   int ns = 0;
+#if PSX_FFT_ORDER == 1
+  for (int i = 0; i < c2 / 3; ++i) r[ns++] = 8;
+  if (c2 % 3 == 1) r[ns++] = 2;
+  if (c2 % 3 == 2) r[ns++] = 4;
+  for (int i = 0; i < c3 / 2; ++i) r[ns++] = 9;
+  if (c3 % 2) r[ns++] = 3;
+  for (int i = 0; i < c5; ++i) r[ns++] = 5;
+#elif PSX_FFT_ORDER == 2
+  for (int i = 0; i < c3 / 2; ++i) r[ns++] = 9;
+  if (c3 % 2) r[ns++] = 3;
+  for (int i = 0; i < c5; ++i) r[ns++] = 5;
+  if (c2 % 3 == 1) r[ns++] = 2;
+  if (c2 % 3 == 2) r[ns++] = 4;
+  for (int i = 0; i < c2 / 3; ++i) r[ns++] = 8;
+#else
   for (int i = 0; i < c5; ++i) r[ns++] = 5;
   for (int i = 0; i < c3 / 2; ++i) r[ns++] = 9;
   if (c3 % 2) r[ns++] = 3;
   if (c2 % 3 == 1) r[ns++] = 2;
   if (c2 % 3 == 2) r[ns++] = 4;
   for (int i = 0; i < c2 / 3; ++i) r[ns++] = 8;
+#endif
   if (ns > FFT_MAX_STAGES) { sh.nstages = -1; return sh; }
   for (int i = 0; i < ns; ++i) sh.radix[i] = r[i];
   sh.nstages = ns;
