@@ -56,7 +56,7 @@ typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 #define PSX_CT_MINW 1
 #endif
 #ifndef PSX_CT_KC_BF
-#define PSX_CT_KC_BF 128
+#define PSX_CT_KC_BF 256
 #endif
 constexpr int CT_MAXLT = 4;   // l tiles in the plan layout (ns <= 64)
 
