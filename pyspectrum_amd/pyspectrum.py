@@ -487,6 +487,8 @@ def _survey_half_on_device(radecz, nb, w, P0_fkp, Lbox, Ngrid, cosmo, prec, devi
         return t
 
     n = radecz.shape[1]
+    if n == 0:
+        raise ValueError("empty catalogue")  # the reference fails in np.max (pyspectrum.py:779)
     rd = dev64(radecz, (3, n))
     nbd = dev64(nb, (n,))
     wd_in = None if w is None else dev64(w, (n,))
