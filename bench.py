@@ -385,7 +385,7 @@ def main():
         xyz_np = xyz.double().cpu().numpy()
         cpu = cpu_baseline(xyz_np, Lbox, N, step, Ncut, Nmax, args.cpu_sample_triangles)
     out = {
-        "metric": "B(k) boxes/s at Ngrid=360 + p50/p99 latency, mixed-precision vs FP64 GPU",
+        "metric": "B(k) boxes/s at Ngrid=%d + p50/p99 latency, mixed-precision vs FP64 GPU" % N,
         "value": round(value, 3), "unit": "boxes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
