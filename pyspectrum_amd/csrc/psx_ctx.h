@@ -36,6 +36,10 @@ struct FftPlan {
 #define PSX_SHELL_X_PAIRS 8
 #endif
 constexpr int SHELL_X_PAIRS = PSX_SHELL_X_PAIRS;
+// LDS row padding (complex values) of the x pass lines.
+#ifndef PSX_XPAD
+#define PSX_XPAD 1
+#endif
 
 // One pruned-pass tile: (shell slot, a, b) with meaning per pass.
 struct Tile3 {

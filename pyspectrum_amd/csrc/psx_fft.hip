@@ -248,7 +248,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
   constexpr int TPL = FFT_THREADS / NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int N = n_of<NC>(sh), lstride = N + 1, hN = N / 2;
+  const int N = n_of<NC>(sh), lstride = N + PSX_XPAD, hN = N / 2;
   const Tile3 t = tiles[blockIdx.x];
   const int s = t.s, z = t.a, y0 = t.b;
   const int kxw = sd.meta[4 * s + 2];
@@ -316,7 +316,7 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
   const int N = p->shape.N;
   ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
   const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
-  const size_t ldsx = ((size_t)SHELL_X_PAIRS * (N + 1) + N) * sizeof(cplx<T>);
+  const size_t ldsx = ((size_t)SHELL_X_PAIRS * (N + PSX_XPAD) + N) * sizeof(cplx<T>);
   const cplx<T>* tw = plan_tw<T>(p);
   if (sp->nz_tiles > 0) {
     PSX_TRY(set_lds(k_shell_z<T, NC>, ldsz));

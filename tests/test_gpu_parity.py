@@ -4,8 +4,8 @@ the reference's own FP64 count caches.
 
 Tolerances (max |gpu - oracle| / max |oracle| unless stated):
   FP64 pipeline   <= 1e-10   (assignment in fixed point, FP64 FFT/MFMA)
-  FP32 pipeline   <= 1e-5    (north-star FP32 bar)
-  split-BF16      <= 1e-3    (north-star split-BF16 bar)
+  FP32 pipeline   <= 1e-5    (the required FP32 accuracy vs FP64)
+  split-BF16      <= 1e-3    (the required split-BF16 accuracy vs FP64)
   bins, counts, N_tri: bit-exact.
 """
 import os
