@@ -7,6 +7,9 @@
   those attributes (an astropy cosmology included) can be passed instead.
 * ``radecz_to_cartesian`` (util.py:27-51), ``ijl_order`` (util.py:8-24) and
   ``applyRSD`` (util.py:54-75).
+* Text outputs of the campaign drivers (run/glam/glam.py:104-106, :127-133):
+  ``write_pk_text``/``read_pk_text`` and ``write_bk_text``/``read_bk_text``
+  with the same columns, formats and header lines.
 """
 import numpy as np
 
@@ -137,3 +140,42 @@ def applyRSD(xyz, vxyz, redshift, h=0.7, omega0_m=0.3, LOS=None, Lbox=None):
     xyz_rsd[i_los] += rsd_factor * vxyz[i_los] + Lbox
     xyz_rsd[i_los] = (xyz_rsd[i_los] % Lbox)
     return xyz_rsd
+
+
+# ------------------------------------------------------------ text outputs --
+PK_TEXT_FMT = '%.5e %.5e %.5e'
+BK_TEXT_FMT = '%i %i %i %.5e %.5e %.5e %.5e %.5e %.5e %.5e'
+BK_TEXT_KEYS = ('i_k1', 'i_k2', 'i_k3', 'p0k1', 'p0k2', 'p0k3', 'b123', 'q123', 'counts', 'b123_sn')
+
+
+def write_pk_text(path, spec, Lbox, nbar):
+    """P_0(k) table of run/glam/glam.py:104-106: columns k, p0k, counts."""
+    kf = 2. * np.pi / Lbox
+    hdr = ('pyspectrum P_l=0(k) calculation. Lbox=%.1f, k_f=%.5e, SN=%.5e' % (Lbox, kf, 1. / nbar))
+    np.savetxt(path, np.array([spec['k'], spec['p0k'], spec['counts']]).T, fmt=PK_TEXT_FMT,
+               delimiter='\t', header=hdr)
+
+
+def read_pk_text(path):
+    """(k, p0k, counts) from a write_pk_text file."""
+    return np.loadtxt(path, skiprows=1, unpack=True, usecols=[0, 1, 2])
+
+
+def write_bk_text(path, bispec, Lbox):
+    """B(k) table of run/glam/glam.py:127-133: i_k, j_k, l_k, p0k1, p0k2, p0k3,
+    b123, q123, counts, b123_sn (the i_k columns must be as long as the
+    others, i.e. every triangle counted)."""
+    cols = [np.asarray(bispec[k]) for k in BK_TEXT_KEYS]
+    if len({len(c) for c in cols}) != 1:
+        raise ValueError("triangle columns of unequal length (a triangle with N_tri = 0)")
+    hdr = 'pyspectrum bispectrum calculation test. k_f = 2pi/%.1f' % Lbox
+    np.savetxt(path, np.array(cols).T, fmt=BK_TEXT_FMT, delimiter='\t', header=hdr)
+
+
+def read_bk_text(path):
+    """Dict of the ten write_bk_text columns (i_k as int64)."""
+    cols = np.loadtxt(path, skiprows=1, unpack=True, usecols=range(10), ndmin=2)
+    out = dict(zip(BK_TEXT_KEYS, cols))
+    for k in ('i_k1', 'i_k2', 'i_k3'):
+        out[k] = out[k].astype(np.int64)
+    return out

@@ -130,3 +130,32 @@ def test_ijl_order_sorts_by_l_then_j_then_i():
     order = util.ijl_order(I, J, Lx).ravel()
     keys = list(zip(Lx[order], J[order], I[order]))
     assert keys == sorted(keys) and len(set(order)) == len(I)
+
+
+def test_driver_text_tables_round_trip(tmp_path):
+    # run/glam/glam.py:104-106, :127-133: same columns, formats and headers
+    rng = np.random.default_rng(19)
+    I, J, Lx, _ = S.triangles(6, 3, 3)
+    bk = {"i_k1": I * 3, "i_k2": J * 3, "i_k3": Lx * 3}
+    for k in ("p0k1", "p0k2", "p0k3", "b123", "q123", "counts", "b123_sn"):
+        bk[k] = rng.standard_normal(len(I)) * 1e4
+    path = tmp_path / "b123.dat"
+    util.write_bk_text(str(path), bk, 2500.0)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "# pyspectrum bispectrum calculation test. k_f = 2pi/2500.0"
+    first = lines[1].split()
+    assert first[:3] == [str(I[0] * 3), str(J[0] * 3), str(Lx[0] * 3)]
+    assert first[3] == "%.5e" % bk["p0k1"][0]
+    back = util.read_bk_text(str(path))
+    assert np.array_equal(back["i_k1"], bk["i_k1"])
+    assert np.allclose(back["b123"], bk["b123"], rtol=1e-5)
+    spec = {"k": np.linspace(0.01, 0.5, 20), "p0k": rng.uniform(1e3, 1e4, 20),
+            "counts": np.arange(20) * 10.0}
+    util.write_pk_text(str(tmp_path / "pk.dat"), spec, 2500.0, 1e-4)
+    hdr = (tmp_path / "pk.dat").read_text().splitlines()[0]
+    assert hdr == "# pyspectrum P_l=0(k) calculation. Lbox=2500.0, k_f=%.5e, SN=1.00000e+04" % (2 * np.pi / 2500)
+    k, p, c = util.read_pk_text(str(tmp_path / "pk.dat"))
+    assert np.allclose(k, spec["k"], rtol=1e-5) and np.array_equal(c, spec["counts"])
+    bk["i_k1"] = bk["i_k1"][:-1]
+    with pytest.raises(ValueError):
+        util.write_bk_text(str(tmp_path / "bad.dat"), bk, 2500.0)
