@@ -69,9 +69,15 @@ template <> struct CtTraits<PSX_PREC_F32> {
   typedef float TD; typedef f4v Acc; typedef double TP;
   static constexpr int KC = 128, MAXPT = 2, PAD = 4;
 };
+#ifndef PSX_CT_PAD_BF
+#define PSX_CT_PAD_BF 4
+#endif
+#ifndef PSX_CT_PADH_BF
+#define PSX_CT_PADH_BF 8
+#endif
 template <> struct CtTraits<PSX_PREC_BF16X3> {
   typedef float TD; typedef f4v Acc; typedef double TP;
-  static constexpr int KC = PSX_CT_KC_BF, MAXPT = 3, PAD = 4;
+  static constexpr int KC = PSX_CT_KC_BF, MAXPT = 3, PAD = PSX_CT_PAD_BF;
 };
 
 // Waves per workgroup and pair tiles per wave.  Split-BF16 up to 3 l tiles:
@@ -91,7 +97,9 @@ template <int MODE, int NLT> __host__ __device__ constexpr int ct_maxpt() {
 template <int MODE> __host__ __device__ constexpr int ct_ldw() {
   return CtTraits<MODE>::KC + CtTraits<MODE>::PAD;
 }
-template <int MODE> __host__ __device__ constexpr int ct_ldwh() { return CtTraits<MODE>::KC + 8; }
+template <int MODE> __host__ __device__ constexpr int ct_ldwh() {
+  return CtTraits<MODE>::KC + PSX_CT_PADH_BF;
+}
 
 template <int MODE>
 static size_t ct_lds_bytes(int ns) {
