@@ -2,9 +2,12 @@
 
 Each ``csrc/*.hip`` is compiled by hipcc into an object under ``_build/``
 and the objects are linked into ``pyspectrum_amd/libpsx.so``.  Packed FP32
-instructions are disabled for all device code.
+instructions are disabled for all device code.  The tile-shape definitions
+chosen by the autotuner (``pyspectrum_amd/tuning.json``, written by
+``python -m pyspectrum_amd.tune``) are applied when present.
 """
 import concurrent.futures as cf
+import json
 import os
 import subprocess
 import sys
@@ -34,13 +37,30 @@ def _run(cmd):
     return err
 
 
-def build(verbose=False, jobs=8):
-    os.makedirs(BUILD, exist_ok=True)
+TUNING = os.path.join(HERE, "tuning.json")
+
+
+def tuned_defines():
+    """-D flags of the autotuner's choice (empty without tuning.json)."""
+    if not os.path.isfile(TUNING):
+        return []
+    with open(TUNING) as f:
+        d = json.load(f).get("defines", {})
+    return ["-D%s=%d" % (k, int(v)) for k, v in sorted(d.items())]
+
+
+def build(verbose=False, jobs=8, defines=None, lib=None, objdir=None):
+    """Compile and link the library; ``defines`` (list of -D flags) default to
+    the tuned ones, ``lib``/``objdir`` to the in-tree paths."""
+    defines = tuned_defines() if defines is None else list(defines)
+    lib = lib or LIB
+    objdir = objdir or BUILD
+    os.makedirs(objdir, exist_ok=True)
     srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
-    objs = [os.path.join(BUILD, s[:-4] + ".o") for s in srcs]
+    objs = [os.path.join(objdir, s[:-4] + ".o") for s in srcs]
 
     def one(src, obj):
-        return _run([HIPCC] + CFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        return _run([HIPCC] + CFLAGS + defines + ["-c", os.path.join(CSRC, src), "-o", obj])
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = [ex.submit(one, s, o) for s, o in zip(srcs, objs)]
@@ -48,10 +68,10 @@ def build(verbose=False, jobs=8):
             msg = f.result()
             if verbose and msg.strip():
                 print(s, msg, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
