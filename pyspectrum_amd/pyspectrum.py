@@ -621,7 +621,7 @@ def B0_survey(radecz, nbar, w=None, radecz_r=None, nbar_r=None, w_r=None, P0_fkp
     _check_ngrid(Ngrid)
     prec = _prec(precision)
     kf = 2 * np.pi / Lbox
-    N = np.shape(radecz)[1]
+    N = radecz.shape[1]
     half_d, Ngtot, _ = _survey_half_on_device(radecz, nbar, w, P0_fkp, Lbox, Ngrid, cosmo, prec,
                                               device)
     half_r, Nrtot, Ir = _survey_half_on_device(radecz_r, nbar_r, w_r, P0_fkp, Lbox, Ngrid, cosmo,
