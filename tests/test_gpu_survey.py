@@ -176,3 +176,24 @@ def test_device_catalogue_matches_host():
     assert np.allclose(sums[:6], exp, rtol=1e-12, atol=0)
     assert np.allclose(sums[7:10], xyz_h.astype(np.float64).max(axis=1), rtol=1e-6)
     assert np.allclose(-sums[10:13], xyz_h.astype(np.float64).min(axis=1), rtol=1e-6)
+
+
+def test_b0_survey_device_tensor_inputs():
+    # catalogues given as CUDA tensors (and unit weights) give the same
+    # result as the numpy inputs: counts and triangles bit for bit, values
+    # that contain the P(k) shell powers (float64 atomic bin sums) to round-off
+    _say("B0_survey device inputs")
+    N, Lbox, Nmax = 32, 1000.0, 5
+    d, nb, _ = _catalogue(3000, 29)
+    r, nbr, _ = _catalogue(9000, 30)
+    kw = dict(P0_fkp=1e4, Lbox=Lbox, Ngrid=N, Nmax=Nmax, precision="fp32")
+    a = P.B0_survey(d.copy(), nb, radecz_r=r.copy(), nbar_r=nbr, **kw)
+    dev = torch.device("cuda", 0)
+    b = P.B0_survey(torch.from_numpy(d).to(dev), torch.from_numpy(nb).to(dev),
+                    radecz_r=torch.from_numpy(r).to(dev), nbar_r=torch.from_numpy(nbr).to(dev), **kw)
+    for k in ("counts", "i_k1", "i_k2", "i_k3"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("p0k1", "b123", "q123"):
+        assert np.allclose(a[k], b[k], rtol=1e-11, atol=1e-11 * np.max(np.abs(a[k]))), k
+    with pytest.raises(ValueError):
+        P.B0_survey(d[:, :0], nb[:0], radecz_r=r, nbar_r=nbr, **kw)
