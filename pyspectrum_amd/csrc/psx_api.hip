@@ -162,8 +162,9 @@ int get_fft_plan(psx_ctx* ctx, int N, const FftPlan** out) {
 
 // ----------------------------------------------------------- shell plan ----
 
-int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, const ShellPlan** out) {
-  auto key = std::make_tuple(N, step, ncut, nmax, kt);
+int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, int xpairs,
+                   const ShellPlan** out) {
+  auto key = std::make_tuple(N, step, ncut, nmax, kt, xpairs);
   auto it = ctx->shell_plans.find(key);
   if (it != ctx->shell_plans.end()) { *out = it->second.get(); return PSX_OK; }
   auto sp = std::make_unique<ShellPlan>();
@@ -196,7 +197,7 @@ int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, co
     for (int z = 0; z < N; ++z)
       for (int kx0 = 0; kx0 < xlim; kx0 += kt) yt.push_back({s, z, kx0, 0});
     for (int z = 0; z < N; ++z)
-      for (int y0 = 0; y0 < N; y0 += 2 * SHELL_X_PAIRS) xt.push_back({s, z, y0, 0});
+      for (int y0 = 0; y0 < N; y0 += 2 * xpairs) xt.push_back({s, z, y0, 0});
   }
   sp->t_elems = off;
   sp->nz_tiles = (int)zt.size();

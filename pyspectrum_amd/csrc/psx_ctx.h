@@ -30,12 +30,42 @@ struct FftPlan {
   double* Wd = nullptr;
 };
 
-// Row pairs (complex FFT lines) per tile of the shell fields' x pass: small
-// tiles keep several workgroups per CU to hide the gather latency.
-#ifndef PSX_SHELL_X_PAIRS
-#define PSX_SHELL_X_PAIRS 8
+// Row pairs (complex FFT lines) per tile of the shell fields' x pass.  Small
+// tiles keep several workgroups per CU to hide the gather latency.  With NP
+// pairs a line gets 256/NP lanes; the radix stages take ceil((N/R)/lanes)
+// passes each, so the lane count decides how many lanes idle: at N = 360
+// (radices 5, 9, 8: 72, 40, 45 butterflies) 25 lanes (10 pairs) waste far
+// less than 32 (8 pairs), at N = 512 (8, 8, 8: 64 each) 32 lanes fit exactly.
+// Float tiles take whichever of 8 or 10 pairs the cost model below prefers
+// (measured: 10 pairs 2.5 % faster shell fields at 360, 13 % slower at 512);
+// double tiles (LDS) and the runtime-N path use 8.  PSX_SHELL_X_PAIRS fixes
+// the float choice for every N.
+#ifndef PSX_SHELL_X_PAIRS_F64
+#define PSX_SHELL_X_PAIRS_F64 8
 #endif
-constexpr int SHELL_X_PAIRS = PSX_SHELL_X_PAIRS;
+// VALU issue of one line's radix stages with `lanes` lanes per line
+// (approximate VALU per butterfly by radix).
+__host__ __device__ constexpr int fft_line_cost(int N, int lanes) {
+  const FftShape sh = fft_shape_of(N);
+  int c = 0;
+  for (int s = 0; s < sh.nstages; ++s) {
+    const int R = sh.radix[s];
+    const int passes = (N / R + lanes - 1) / lanes;
+    const int cr = R == 9 ? 140 : R == 8 ? 90 : R == 5 ? 60 : R == 4 ? 35 : R == 3 ? 25 : 10;
+    c += passes * cr;
+  }
+  return c * lanes;
+}
+// NC: compile-time grid size, 0 for the runtime-N path.
+template <typename T> __host__ __device__ constexpr int shell_x_pairs(int NC) {
+  if (sizeof(T) != 4) return PSX_SHELL_X_PAIRS_F64;
+#ifdef PSX_SHELL_X_PAIRS
+  return PSX_SHELL_X_PAIRS;
+#else
+  if (NC <= 0) return 8;
+  return fft_line_cost(NC, FFT_THREADS / 10) < fft_line_cost(NC, FFT_THREADS / 8) ? 10 : 8;
+#endif
+}
 // LDS row padding (complex values) of the x pass lines.
 #ifndef PSX_XPAD
 #define PSX_XPAD 1
@@ -85,7 +115,7 @@ struct psx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::map<int, psx::FftPlan> fft_plans;
-  std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<psx::ShellPlan>> shell_plans;
+  std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<psx::ShellPlan>> shell_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<psx::ContractPlan>> contract_plans;
   std::map<std::string, psx::DevBuf> bufs;
   std::vector<void*> owned;  // plan allocations
@@ -96,7 +126,8 @@ namespace psx {
 int ctx_buffer(psx_ctx* ctx, const char* name, size_t bytes, void** out);
 int ctx_alloc_owned(psx_ctx* ctx, size_t bytes, void** out);
 int get_fft_plan(psx_ctx* ctx, int N, const FftPlan** out);
-int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, const ShellPlan** out);
+int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, int xpairs,
+                   const ShellPlan** out);
 int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
                       const ContractPlan** out);
 

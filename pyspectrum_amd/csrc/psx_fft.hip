@@ -232,7 +232,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
 }
 
 // x pass (Hermitian -> real): tile (s, z, y0) covers rows y0..y0+2NP-1 as NP
-// pairs (NP = SHELL_X_PAIRS); each pair is one complex FFT of
+// pairs (NP = shell_x_pairs<T>(NC)); each pair is one complex FFT of
 // X1 + i X2 whose real/imag parts are the two real rows.  kx = 0 and kx = N/2
 // keep only their real part, which makes the result exactly Re FFT of the
 // Hermitian extension.  Thread (p, lane) works on pair p throughout, so the
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
                                                          ShellDev sd, FftShape sh,
                                                          const cplx<T>* __restrict__ tw,
                                                          bool vec4) {
-  constexpr int NP = SHELL_X_PAIRS;
+  constexpr int NP = shell_x_pairs<T>(NC);
   constexpr int TPL = FFT_THREADS / NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -316,7 +316,7 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
   const int N = p->shape.N;
   ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
   const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
-  const size_t ldsx = ((size_t)SHELL_X_PAIRS * (N + PSX_XPAD) + N) * sizeof(cplx<T>);
+  const size_t ldsx = ((size_t)shell_x_pairs<T>(NC) * (N + PSX_XPAD) + N) * sizeof(cplx<T>);
   const cplx<T>* tw = plan_tw<T>(p);
   if (sp->nz_tiles > 0) {
     PSX_TRY(set_lds(k_shell_z<T, NC>, ldsz));
@@ -349,7 +349,12 @@ static int shell_fields_t(psx_ctx* ctx, const cplx<T>* H, int N, int step, int n
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
   const ShellPlan* sp;
-  PSX_TRY(get_shell_plan(ctx, N, step, ncut, nmax, TileCfg<T>::LINES, &sp));
+  // the x tiles follow the pair count of the kernel instantiation used for N
+  int nc = 0;
+#define PSX_IF_N(n) if (N == n) nc = n;
+  PSX_FFT_SIZES(PSX_IF_N)
+#undef PSX_IF_N
+  PSX_TRY(get_shell_plan(ctx, N, step, ncut, nmax, TileCfg<T>::LINES, shell_x_pairs<T>(nc), &sp));
   void *t1 = nullptr, *t2 = nullptr;
   PSX_TRY(ctx_buffer(ctx, sizeof(T) == 4 ? "shell_t1_f32" : "shell_t1_f64",
                      (size_t)sp->t_elems * sizeof(cplx<T>), &t1));

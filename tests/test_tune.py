@@ -15,7 +15,9 @@ def test_candidates_name_real_knobs():
         if f.endswith((".hip", ".h")):
             with open(os.path.join(B.CSRC, f)) as fh:
                 src += fh.read()
+    # knobs with a default (#ifndef X / #define X v) and optional overrides (#ifdef X)
     knobs = set(re.findall(r"#ifndef (PSX_[A-Z0-9_]+)\n#define \1 ", src))
+    knobs |= set(re.findall(r"#ifdef (PSX_[A-Z0-9_]+)\n", src))
     assert "default" in T.CANDIDATES and T.CANDIDATES["default"] == {}
     for name, d in T.CANDIDATES.items():
         assert set(d) <= knobs, (name, set(d) - knobs)
