@@ -74,7 +74,13 @@ struct AssignTiling {
 
 // At most 8 lines per tile keeps the LDS of a tile (16 B per cell plus the
 // complex lines) small enough for several workgroups per CU.
-constexpr int ASSIGN_LINES = 8;
+#ifndef PSX_ASSIGN_LINES
+#define PSX_ASSIGN_LINES 8
+#endif
+#ifndef PSX_ASSIGN_TY
+#define PSX_ASSIGN_TY 4
+#endif
+constexpr int ASSIGN_LINES = PSX_ASSIGN_LINES;
 template <typename T, int NC>
 constexpr int assign_lines_max() {
   return Tile<T, NC>::MAXLINES < ASSIGN_LINES ? Tile<T, NC>::MAXLINES : ASSIGN_LINES;
@@ -84,7 +90,7 @@ template <typename T>
 static AssignTiling assign_tiling(int N) {
   const int tl = std::min(tile_lines_host<T>(N), ASSIGN_LINES);
   AssignTiling a;
-  a.ty = tl >= 4 ? 4 : tl;
+  a.ty = tl >= PSX_ASSIGN_TY ? PSX_ASSIGN_TY : tl;
   a.tz = tl / a.ty;
   a.nty = (N + a.ty - 1) / a.ty;
   a.ntz = (N + a.tz - 1) / a.tz;
