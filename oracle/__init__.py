@@ -8,14 +8,17 @@ anything here and fails loudly when its HIP library is missing.
 Contents
 --------
 ``spectrum``  FP64 numpy restatement of the reference's periodic-box path
-              (``pyspectrum/pyspectrum.py`` and ``pyspectrum/estimator.f``);
-              every function cites the reference lines it follows.
+              and survey monopole (``pyspectrum/pyspectrum.py``,
+              ``pyspectrum/estimator.f``, ``pyspectrum/util.py``); every
+              function cites the reference lines it follows.
 ``ref``       ctypes binding to ``oracle/_ref/libestimator.so``, the reference's
               own Fortran kernels (``assign_quad``, ``fcomb_periodic``,
-              ``pk_pbox_rsd``) compiled from ``/root/reference`` by
-              ``oracle/Makefile``.  Used only to pin ``spectrum`` in tests.
+              ``fcomb_survey``, ``pk_pbox_rsd``) compiled from
+              ``/root/reference`` by ``oracle/Makefile``.  Used only to pin
+              ``spectrum`` in tests.
 
-Pins (how each oracle function is checked; see ``tests/test_oracle.py``):
+Pins (how each oracle function is checked; see ``tests/test_oracle.py`` and
+``tests/test_survey_oracle.py``):
 
 * ``assign_quad``       -- the reference Fortran (``ref``), mass/normalisation
                            invariants, the 1-4-1 node stencil.
@@ -30,4 +33,11 @@ Pins (how each oracle function is checked; see ``tests/test_oracle.py``):
 * ``bk_from_shells``    -- direct k-space triple sum on tiny grids.
 * ``pk_pbox_rsd``       -- the reference Fortran (``ref``), bit-exact bins.
 * ``Pk_periodic``       -- Parseval identity against the shell fields.
+* ``assign_quad(offset)`` -- the reference Fortran with offset Ngrid/2 and
+                           negative (observer-centred) positions.
+* ``fcomb_survey``      -- the reference Fortran ``fcomb_survey``.
+* ``comoving_distance`` -- closed forms (Omega_m = 1 and 0).
+* ``radecz_to_cartesian`` -- axis points.
+* ``B0_survey_from_sums`` -- closed form when data = randoms (delta = 0).
+* ``FFT_survey_mono`` / ``B0_survey`` -- compositions of the pinned pieces.
 """
