@@ -89,7 +89,13 @@ def call(name, *args):
 
 
 class Context:
-    """One library context on one device; work goes to torch's current stream."""
+    """One library context on one device; work goes to torch's current stream.
+
+    ``call(name, *args)`` binds the calling thread's current stream and runs
+    the entry point under the context's lock, so threads may share a context:
+    their calls are serialised, and the library orders each stream switch
+    after the work already queued on the previous stream (the workspace is
+    never used by two streams at once)."""
 
     def __init__(self, device=0):
         import torch
@@ -97,15 +103,22 @@ class Context:
             raise PsxError("no GPU visible: the HIP path has no CPU fallback")
         self.device = device
         self._h = _p()
+        self._lock = threading.RLock()
         with torch.cuda.device(device):
             call("psx_ctx_create", ctypes.byref(self._h), device)
 
     @property
     def handle(self):
+        """Bind the current stream and return the raw handle (single-threaded use)."""
         import torch
         stream = torch.cuda.current_stream(self.device).cuda_stream
         call("psx_ctx_set_stream", self._h, _p(stream))
         return self._h
+
+    def call(self, name, *args):
+        """``name(ctx, *args)`` on the calling thread's current stream."""
+        with self._lock:
+            return call(name, self.handle, *args)
 
     def workspace_bytes(self):
         n = _i64()
@@ -119,13 +132,16 @@ class Context:
 
 
 _contexts = {}
+_contexts_lock = threading.Lock()
 
 
 def context(device=0):
-    ctx = _contexts.get(device)
-    if ctx is None:
-        ctx = _contexts[device] = Context(device)
-    return ctx
+    """The process-wide default context of a device (created once)."""
+    with _contexts_lock:
+        ctx = _contexts.get(device)
+        if ctx is None:
+            ctx = _contexts[device] = Context(device)
+        return ctx
 
 
 def ptr(t):

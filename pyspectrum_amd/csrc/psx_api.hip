@@ -28,6 +28,9 @@ int launch_bk_finalize(psx_ctx* ctx, const double* braw, const double* p0k, cons
                        double a6, double b6, double* out);
 int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k);
 
+// Serialises the entry points on one context (host-side plan and workspace maps).
+#define PSX_CTX_LOCK(c) std::lock_guard<std::recursive_mutex> psx_ctx_lock_((c)->mu)
+
 std::string& last_error_string() {
   static thread_local std::string s;
   return s;
@@ -497,6 +500,11 @@ int psx_ctx_create(psx_ctx** out, int device) {
   PSX_HIP(hipSetDevice(device));
   psx_ctx* c = new psx_ctx();
   c->device = device;
+  const hipError_t e = hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    delete c;
+    return set_error(PSX_ERR_HIP, std::string("hipEventCreateWithFlags: ") + hipGetErrorString(e));
+  }
   *out = c;
   return PSX_OK;
 }
@@ -507,24 +515,35 @@ int psx_ctx_destroy(psx_ctx* ctx) {
   (void)hipDeviceSynchronize();
   for (auto& kv : ctx->bufs) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (void* p : ctx->owned) (void)hipFree(p);
+  if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
   delete ctx;
   return PSX_OK;
 }
 
 int psx_ctx_set_stream(psx_ctx* ctx, void* stream) {
   PSX_REQUIRE(ctx, "null ctx");
-  ctx->stream = (hipStream_t)stream;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  const hipStream_t s = (hipStream_t)stream;
+  if (s != ctx->stream) {
+    // work queued on the old stream (which may use the workspace) completes
+    // before anything the new stream runs after this point
+    PSX_HIP(hipEventRecord(ctx->order_ev, ctx->stream));
+    PSX_HIP(hipStreamWaitEvent(s, ctx->order_ev, 0));
+    ctx->stream = s;
+  }
   return PSX_OK;
 }
 
 int psx_ctx_synchronize(psx_ctx* ctx) {
   PSX_REQUIRE(ctx, "null ctx");
+  PSX_CTX_LOCK(ctx);
   PSX_HIP(hipStreamSynchronize(ctx->stream));
   return PSX_OK;
 }
 
 int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes) {
   PSX_REQUIRE(ctx && bytes, "null argument");
+  PSX_CTX_LOCK(ctx);
   int64_t s = 0;
   for (auto& kv : ctx->bufs) s += (int64_t)kv.second.bytes;
   *bytes = s;
@@ -535,6 +554,7 @@ int psx_assign_quad(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, 
                     float kf_ks, float offset, float clip_lo, float clip_hi, double wsum_abs,
                     void* grid_out, int prec) {
   PSX_REQUIRE(ctx && grid_out && (xyz || np == 0), "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_REQUIRE(np >= 0, "negative particle count");
   PSX_REQUIRE(clip_lo <= clip_hi && kf_ks > 0.0f, "bad clip range or kf_ks");
@@ -551,6 +571,7 @@ int psx_assign_quad(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, 
 
 int psx_fft3d(psx_ctx* ctx, void* grid, int ngrid, int sign, int prec) {
   PSX_REQUIRE(ctx && grid, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_REQUIRE(sign == 1 || sign == -1, "sign must be +1 or -1");
   PSX_TRY(check_ngrid(ngrid));
@@ -560,6 +581,7 @@ int psx_fft3d(psx_ctx* ctx, void* grid, int ngrid, int sign, int prec) {
 int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid, double sumw,
                        int prec) {
   PSX_REQUIRE(ctx && grid && half_out, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_ngrid(ngrid));
   const FftPlan* ft;
@@ -570,6 +592,7 @@ int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid
 int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                      double lbox, double sumw, double wsum_abs, void* half_out, int prec) {
   PSX_REQUIRE(ctx && half_out && (xyz || np == 0), "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_REQUIRE(np >= 0 && lbox > 0, "bad particle count or box size");
   PSX_TRY(check_ngrid(ngrid));
@@ -581,6 +604,7 @@ int psx_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, c
                          double zmax, double h, double p0_fkp, float* xyz_out, float* w_out,
                          double* sums_out) {
   PSX_REQUIRE(ctx && sums_out && dist_nodes && dist_slopes, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(n == 0 || (radecz && nb && xyz_out && w_out), "null argument");
   PSX_REQUIRE(n >= 0 && nseg >= 1 && zmax >= 0.0, "bad object count, table size or zmax");
   return launch_survey_catalogue(ctx, radecz, nb, w, n, dist_nodes, dist_slopes, nseg, zmax, h,
@@ -590,6 +614,7 @@ int psx_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, c
 int psx_fft_survey_mono(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                         double lbox, double wsum_abs, void* half_out, int prec) {
   PSX_REQUIRE(ctx && half_out && (xyz || np == 0), "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_REQUIRE(np >= 0 && lbox > 0, "bad particle count or box size");
   PSX_TRY(check_ngrid(ngrid));
@@ -605,6 +630,7 @@ int psx_fft_survey_mono(psx_ctx* ctx, const float* xyz, const float* w, int64_t 
 
 int psx_reflect_delta(psx_ctx* ctx, const void* half, void* full_out, int ngrid, int prec) {
   PSX_REQUIRE(ctx && half && full_out, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_ngrid(ngrid));
   return launch_reflect(ctx, half, full_out, ngrid, fft_prec(prec));
@@ -613,6 +639,7 @@ int psx_reflect_delta(psx_ctx* ctx, const void* half, void* full_out, int ngrid,
 int psx_hermitian_half(psx_ctx* ctx, const void* full, int in_prec, int64_t sx, int64_t sy,
                        int64_t sz, void* half_out, int ngrid, int prec) {
   PSX_REQUIRE(ctx && full && half_out, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec) && prec_ok(in_prec), "unknown precision");
   PSX_TRY(check_ngrid(ngrid));
   return launch_herm_half(ctx, full, fft_prec(in_prec), sx, sy, sz, half_out, ngrid, fft_prec(prec));
@@ -621,6 +648,7 @@ int psx_hermitian_half(psx_ctx* ctx, const void* full, int in_prec, int64_t sx, 
 int psx_bin_power(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins, double kf,
                   double* out3, int prec) {
   PSX_REQUIRE(ctx && half && out3, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_REQUIRE(step >= 1, "step >= 1 required");
   PSX_TRY(check_ngrid(ngrid));
@@ -630,6 +658,7 @@ int psx_bin_power(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins
 int psx_pk_pbox_rsd(psx_ctx* ctx, const void* half, int irsd, int lbox, int nbin, int nmu,
                     int ngrid, double* out, int prec) {
   PSX_REQUIRE(ctx && half && out, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_ngrid(ngrid));
   return launch_pk_rsd(ctx, half, ngrid, irsd, lbox, nbin, nmu, fft_prec(prec), out);
@@ -656,6 +685,7 @@ int psx_bk_triangles(int step, int ncut, int nmax, int32_t* out) {
 int psx_bk_shell_fields(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
                         void* d_out, int prec) {
   PSX_REQUIRE(ctx && half && d_out, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
   return launch_shell_fields(ctx, half, ngrid, step, ncut, nmax, fft_prec(prec), d_out);
@@ -664,6 +694,7 @@ int psx_bk_shell_fields(psx_ctx* ctx, const void* half, int ngrid, int step, int
 int psx_bk_contract(psx_ctx* ctx, const void* d, int ngrid, int step, int ncut, int nmax,
                     double* b_raw, int prec) {
   PSX_REQUIRE(ctx && d && b_raw, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
   return launch_contract(ctx, d, ngrid, step, ncut, nmax, prec, b_raw);
@@ -672,6 +703,7 @@ int psx_bk_contract(psx_ctx* ctx, const void* d, int ngrid, int step, int ncut, 
 int psx_bk_raw(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
                double* b_raw, double* p0k, int prec) {
   PSX_REQUIRE(ctx && half && b_raw, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
   return bk_raw_impl(ctx, half, ngrid, step, ncut, nmax, b_raw, p0k, prec);
@@ -679,6 +711,7 @@ int psx_bk_raw(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, in
 
 int psx_bk_counts(psx_ctx* ctx, int ngrid, int step, int ncut, int nmax, double* counts) {
   PSX_REQUIRE(ctx && counts, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
   const int64_t Nh = ngrid / 2 + 1;
   const int64_t nh = Nh * ngrid * ngrid;
@@ -692,6 +725,7 @@ int psx_bk_finalize(psx_ctx* ctx, const double* b_raw, const double* p0k, const 
                     const int32_t* ijl, int ntri, int ngrid, double nbar, double a3, double b3,
                     double a6, double b6, double* out7) {
   PSX_REQUIRE(ctx && b_raw && p0k && counts && ijl && out7, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(ntri >= 0, "negative triangle count");
   if (ntri == 0) return PSX_OK;
   const double G = (double)ngrid * (double)ngrid * (double)ngrid;
@@ -703,6 +737,7 @@ int psx_bk_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, 
                     const double* counts, const int32_t* ijl, double nbar, double a3, double b3,
                     double a6, double b6, double* out7, double* p0k, int prec) {
   PSX_REQUIRE(ctx && (xyz || np == 0) && counts && ijl && out7 && p0k, "null argument");
+  PSX_CTX_LOCK(ctx);
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
   const int fp = fft_prec(prec);

@@ -2,6 +2,7 @@
 #pragma once
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -114,6 +115,11 @@ struct ContractPlan {
 struct psx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Entry points hold the lock; a stream switch makes the new stream wait on
+  // an event recorded on the old one, so the workspace is never in use on two
+  // streams at once.
+  std::recursive_mutex mu;
+  hipEvent_t order_ev = nullptr;
   std::map<int, psx::FftPlan> fft_plans;
   std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<psx::ShellPlan>> shell_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<psx::ContractPlan>> contract_plans;

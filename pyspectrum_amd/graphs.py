@@ -47,8 +47,8 @@ class BkGraph:
         torch.cuda.synchronize(dev)
 
     def _launch(self):
-        L.call("psx_bk_raw", self.ctx.handle, L.ptr(self.half), self.N, self.step, self.Ncut,
-               self.Nmax, L.ptr(self.braw), L.ptr(self.p0k), self.prec)
+        self.ctx.call("psx_bk_raw", L.ptr(self.half), self.N, self.step, self.Ncut, self.Nmax,
+                      L.ptr(self.braw), L.ptr(self.p0k), self.prec)
 
     def replay(self):
         """Run the captured sequence on the current stream (reads ``half``)."""

@@ -98,7 +98,7 @@ def _half_on_device(xyz, w, Lbox, Ngrid, prec, device):
     wd, sumw, sumabs = _weights(w, n, device)
     Nh = Ngrid // 2 + 1
     half = torch.empty((Ngrid, Ngrid, Nh), dtype=_cdtype(prec), device=_device(device))
-    L.call("psx_fft_periodic", ctx.handle, L.ptr(x), L.ptr(wd), n, Ngrid, float(Lbox), sumw, sumabs,
+    ctx.call("psx_fft_periodic", L.ptr(x), L.ptr(wd), n, Ngrid, float(Lbox), sumw, sumabs,
            L.ptr(half), prec)
     return half, n, sumw
 
@@ -147,7 +147,7 @@ def reflect_delta(delt, Ngrid=360, silent=True, device=0):
         raise ValueError("delt must have shape (Ngrid//2+1, Ngrid, Ngrid)")
     half = d.to(device=_device(device), dtype=_cdtype(prec)).permute(2, 1, 0).contiguous()
     full = torch.empty((Ngrid, Ngrid, Ngrid), dtype=_cdtype(prec), device=_device(device))
-    L.call("psx_reflect_delta", L.context(device).handle, L.ptr(half), L.ptr(full), Ngrid, prec)
+    L.context(device).call("psx_reflect_delta", L.ptr(half), L.ptr(full), Ngrid, prec)
     return full.cpu().numpy()
 
 
@@ -174,7 +174,7 @@ def Pk_periodic(xyz, w=None, Lbox=2600, Ngrid=360, fft='pyfftw', silent=True, pr
     Nbins = Ngrid // 2
     kf = 2 * np.pi / float(Lbox)
     sums = torch.empty(3 * Nbins, dtype=torch.float64, device=_device(device))
-    L.call("psx_bin_power", L.context(device).handle, L.ptr(half), Ngrid, 1, Nbins, kf,
+    L.context(device).call("psx_bin_power", L.ptr(half), Ngrid, 1, Nbins, kf,
            L.ptr(sums), prec)
     ksum, psum, nk = sums.cpu().numpy().reshape(3, Nbins)
     k = np.zeros(Nbins)
@@ -225,7 +225,7 @@ def _pk_pbox_rsd(half, irsd, Lbox, Nbin, Nmu, Ngrid, prec, device, ctx=None):
     """pk_pbox_rsd on a device half field; the final divisions as estimator.f:246-263."""
     ctx = ctx or L.context(device)
     out = torch.empty(5 * Nbin + 4 * Nbin * Nmu, dtype=torch.float64, device=_device(device))
-    L.call("psx_pk_pbox_rsd", ctx.handle, L.ptr(half), int(irsd), int(Lbox), Nbin,
+    ctx.call("psx_pk_pbox_rsd", L.ptr(half), int(irsd), int(Lbox), Nbin,
            Nmu, Ngrid, L.ptr(out), prec)
     o = out.cpu().numpy()
     k, p0, p2, p4, nk = [o[i * Nbin:(i + 1) * Nbin].copy() for i in range(5)]
@@ -316,7 +316,7 @@ def _counts_Bk123(Ngrid=360, Nmax=40, Ncut=3, step=3, fft='pyfftw', silent=True,
 def _counts_on_gpu(Ngrid, Nmax, Ncut, step, device=0):
     ijl = L.triangles(step, Ncut, Nmax)
     out = torch.zeros(max(len(ijl), 1), dtype=torch.float64, device=_device(device))
-    L.call("psx_bk_counts", L.context(device).handle, Ngrid, step, Ncut, Nmax, L.ptr(out))
+    L.context(device).call("psx_bk_counts", Ngrid, step, Ncut, Nmax, L.ptr(out))
     vals = out.cpu().numpy()[: len(ijl)]
     counts = np.zeros((Nmax, Nmax, Nmax), dtype=np.float64)
     counts[ijl[:, 0] - 1, ijl[:, 1] - 1, ijl[:, 2] - 1] = vals
@@ -376,7 +376,7 @@ def Bk_periodic(xyz, w=None, Lbox=2600, Ngrid=360, step=3, Ncut=3, Nmax=40, fft=
     ijl_d = torch.from_numpy(np.ascontiguousarray(ijl)).to(dev)
     out7 = torch.empty((7, max(ntri, 1)), dtype=torch.float64, device=dev)
     p0k = torch.empty(Nmax, dtype=torch.float64, device=dev)
-    L.call("psx_bk_periodic", ctx.handle, L.ptr(x), L.ptr(wd), N, Ngrid, float(Lbox), sw, sabs, step,
+    ctx.call("psx_bk_periodic", L.ptr(x), L.ptr(wd), N, Ngrid, float(Lbox), sw, sabs, step,
            Ncut, Nmax, L.ptr(cnt_d), L.ptr(ijl_d), float(nbar), (2 * np.pi) ** 3, kf ** 3,
            (2 * np.pi) ** 6, kf ** 6, L.ptr(out7), L.ptr(p0k), prec)
     fin = out7.cpu().numpy()[:, :ntri]
@@ -406,13 +406,13 @@ def _Bk_periodic(delta, Nmax=40, Ncut=3, step=3, fft='pyfftw', nthreads=1, silen
     sx, sy, sz = full.stride()
     Nh = Ngrid // 2 + 1
     half = torch.empty((Ngrid, Ngrid, Nh), dtype=_cdtype(prec), device=dev)
-    L.call("psx_hermitian_half", ctx.handle, L.ptr(full), in_prec, sx, sy, sz, L.ptr(half), Ngrid,
+    ctx.call("psx_hermitian_half", L.ptr(full), in_prec, sx, sy, sz, L.ptr(half), Ngrid,
            prec)
     ijl = L.triangles(step, Ncut, Nmax)
     ntri = len(ijl)
     braw = torch.empty(max(ntri, 1), dtype=torch.float64, device=dev)
     p0k_d = torch.empty(Nmax, dtype=torch.float64, device=dev)
-    L.call("psx_bk_raw", ctx.handle, L.ptr(half), Ngrid, step, Ncut, Nmax, L.ptr(braw), L.ptr(p0k_d),
+    ctx.call("psx_bk_raw", L.ptr(half), Ngrid, step, Ncut, Nmax, L.ptr(braw), L.ptr(p0k_d),
            prec)
     counts = _counts_Bk123(Ngrid=Ngrid, Nmax=Nmax, Ncut=Ncut, step=step, fft=fft, silent=silent,
                            device=device)
@@ -501,7 +501,7 @@ def _survey_half_on_device(radecz, nb, w, P0_fkp, Lbox, Ngrid, cosmo, prec, devi
     wf = torch.empty(n, dtype=torch.float32, device=dev)
     sums = np.zeros(L.SURVEY_SUMS)
     ctx = L.context(device)
-    L.call("psx_survey_catalogue", ctx.handle, L.ptr(rd), L.ptr(nbd), L.ptr(wd_in), n,
+    ctx.call("psx_survey_catalogue", L.ptr(rd), L.ptr(nbd), L.ptr(wd_in), n,
            Dn.ctypes.data_as(L._p), sl.ctypes.data_as(L._p), len(Dn) - 1, zmax,
            float(_value_h(cosmo)), float(P0_fkp), L.ptr(xyz), L.ptr(wf),
            sums.ctypes.data_as(L._p))
@@ -511,7 +511,7 @@ def _survey_half_on_device(radecz, nb, w, P0_fkp, Lbox, Ngrid, cosmo, prec, devi
         'box not big enough!'
     assert np.all([(I12 >= 0), (I13 >= 0), (I22 >= 0), (I23 >= 0), (I33 >= 0)])
     half = torch.empty((Ngrid, Ngrid, Ngrid // 2 + 1), dtype=_cdtype(prec), device=dev)
-    L.call("psx_fft_survey_mono", ctx.handle, L.ptr(xyz), L.ptr(wf), n, Ngrid, float(Lbox), sabs,
+    ctx.call("psx_fft_survey_mono", L.ptr(xyz), L.ptr(wf), n, Ngrid, float(Lbox), sabs,
            L.ptr(half), prec)
     return half, Ntot, (I12, I13, I22, I23, I33)
 
@@ -550,7 +550,7 @@ def _B0_from_half(half, alpha, I12, I13, I22, I23, I33, Nmax, Ncut, step, fft, s
     ntri = len(ijl)
     braw_d = torch.empty(max(ntri, 1), dtype=torch.float64, device=dev)
     p0k_d = torch.empty(Nmax, dtype=torch.float64, device=dev)
-    L.call("psx_bk_raw", L.context(device).handle, L.ptr(half), Ngrid, step, Ncut, Nmax,
+    L.context(device).call("psx_bk_raw", L.ptr(half), Ngrid, step, Ncut, Nmax,
            L.ptr(braw_d), L.ptr(p0k_d), prec)
     counts = _counts_Bk123(Ngrid=Ngrid, Nmax=Nmax, Ncut=Ncut, step=step, fft=fft, silent=silent,
                            device=device)
@@ -595,7 +595,7 @@ def _B0_survey(delta, alpha, I12, I13, I22, I23, I33, Nmax=40, Ncut=3, step=3, f
     full = d.to(device=dev, dtype=_cdtype(in_prec))
     sx, sy, sz = full.stride()
     half = torch.empty((Ngrid, Ngrid, Ngrid // 2 + 1), dtype=_cdtype(prec), device=dev)
-    L.call("psx_hermitian_half", L.context(device).handle, L.ptr(full), in_prec, sx, sy, sz,
+    L.context(device).call("psx_hermitian_half", L.ptr(full), in_prec, sx, sy, sz,
            L.ptr(half), Ngrid, prec)
     return _B0_from_half(half, alpha, I12, I13, I22, I23, I33, Nmax, Ncut, step, fft, silent, prec,
                          device)

@@ -80,12 +80,12 @@ class Slot:
         N, n = req.Ngrid, req.xyz.shape[1]
         graph = self.graphs.get((N, req.step, req.Ncut, req.Nmax, req.precision))
         with torch.cuda.stream(self.stream):
-            h = self.ctx.handle  # binds this slot's stream
             cd = torch.complex128 if req.precision == "fp64" else torch.complex64
             half = graph.half if graph is not None else torch.empty((N, N, N // 2 + 1), dtype=cd,
                                                                     device=dev)
-            L.call("psx_fft_periodic", h, L.ptr(req.xyz), None, n, N, float(req.Lbox), float(n),
-                   float(n), L.ptr(half), prec)
+            # each call binds this slot's stream (the current stream) to its context
+            self.ctx.call("psx_fft_periodic", L.ptr(req.xyz), None, n, N, float(req.Lbox),
+                          float(n), float(n), L.ptr(half), prec)
             pk = P._pk_pbox_rsd(half, 2, req.Lbox, N // 2, req.Nmubin, N, prec, self.device,
                                  ctx=self.ctx)
             ijl = L.triangles(req.step, req.Ncut, req.Nmax)
@@ -99,14 +99,14 @@ class Slot:
             else:
                 braw = torch.empty(max(ntri, 1), dtype=torch.float64, device=dev)
                 p0k = torch.empty(req.Nmax, dtype=torch.float64, device=dev)
-                L.call("psx_bk_raw", self.ctx.handle, L.ptr(half), N, req.step, req.Ncut,
+                self.ctx.call("psx_bk_raw", L.ptr(half), N, req.step, req.Ncut,
                        req.Nmax, L.ptr(braw), L.ptr(p0k), prec)
             nbar = n / req.Lbox ** 3
             kf = 2 * np.pi / req.Lbox
             cnt = torch.from_numpy(tc).to(dev)
             ijl_d = torch.from_numpy(np.ascontiguousarray(ijl)).to(dev)
             out7 = torch.empty((7, max(ntri, 1)), dtype=torch.float64, device=dev)
-            L.call("psx_bk_finalize", self.ctx.handle, L.ptr(braw), L.ptr(p0k), L.ptr(cnt),
+            self.ctx.call("psx_bk_finalize", L.ptr(braw), L.ptr(p0k), L.ptr(cnt),
                    L.ptr(ijl_d), ntri, N, float(nbar), (2 * np.pi) ** 3, kf ** 3, (2 * np.pi) ** 6,
                    kf ** 6, L.ptr(out7))
             fin = out7.cpu().numpy()[:, :ntri]   # synchronises this stream
