@@ -46,7 +46,11 @@ int psx_abi_version(void);
 /* Message of the last failing call on this thread ("" if none). */
 const char* psx_last_error(void);
 
-/* Context: one per device and stream; plans and workspace are cached in it.
+/* Context: one per device; plans and workspace are cached in it, work goes to
+ * the stream last set.  Entry points lock the context, and setting a new
+ * stream makes it wait for the work queued on the previous one, so threads
+ * may share a context (set the stream and call under one caller-side lock
+ * to keep each call on the caller's stream).
  * (No reference counterpart: the reference keeps no state between calls.) */
 int psx_ctx_create(psx_ctx** ctx, int device);
 int psx_ctx_destroy(psx_ctx* ctx);
