@@ -387,3 +387,31 @@ def test_precision_stress_config(nmax, ntri):
         assert np.max(np.abs(braw - braw64) / np.abs(braw64)) < tol_raw, p
         assert np.max(np.abs(got["b123"][keep] - ref["b123"][keep]) / den) < tol, p
         assert _rel(got["p0k1"], ref["p0k1"]) < tol, p
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_bk_periodic_weighted_matches_oracle(prec):
+    # weights enter the assignment, sum w (fcomb normalisation) and nbar
+    # (shot noise), pyspectrum.py:312-349
+    _say("Bk_periodic weighted", prec)
+    N, Lbox = 48, 600.0
+    xyz, w = _particles(15000, Lbox, 33, weights=True)
+    counts = S.counts_Bk123(N, 8, 3, 3)
+    counts[np.rint(counts / N ** 3) == 0] = 0.0
+    ref = S.Bk_periodic(xyz, w, Lbox, N, 3, 3, 8, counts=counts)
+    got = P.Bk_periodic(xyz, w=w, Lbox=Lbox, Ngrid=N, step=3, Ncut=3, Nmax=8, precision=prec)
+    _bk_compare(got, ref, TOL[prec])
+    assert got["meta"]["nbar"] == pytest.approx(np.sum(w) / Lbox ** 3, rel=1e-14)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pk_periodic_rsd_weighted_matches_oracle(prec):
+    _say("Pk_periodic_rsd weighted", prec)
+    N, Lbox = 40, 700
+    xyz, w = _particles(15000, Lbox, 34, weights=True)
+    ref = S.Pk_periodic_rsd(xyz, w, Lbox, N, 2, 10)
+    got = P.Pk_periodic_rsd(xyz, w=w, Lbox=Lbox, Ngrid=N, rsd=2, Nmubin=10, precision=prec)
+    assert np.array_equal(got["counts"], ref["counts"])
+    tol = 1e-9 if prec == "fp64" else 1e-5
+    for k in ("k", "p0k", "p_kmu"):
+        assert _rel(got[k], ref[k]) < tol, k
