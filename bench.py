@@ -266,6 +266,9 @@ def run_stream(args):
                 s.run(w)
         torch.cuda.synchronize()
         out, wall = server.serve(reqs)
+        failed = [r for r in out if r.error is not None]
+        if failed:
+            raise SystemExit("request %d failed: %s" % (failed[0].rid, failed[0].error))
         res = (len(out) / wall, serve.latency_stats(out), wall, server.graph_build_s)
         server.close()
         return res

@@ -53,6 +53,7 @@ class Request:
     Nmubin: int = 10
     precision: str = "bf16x3"
     result: dict = field(default=None, repr=False)
+    error: str = None            # set instead of result when the request failed
     t_start: float = float("nan")
     t_done: float = float("nan")
 
@@ -161,7 +162,11 @@ class Server:
         reqs = sorted(requests, key=lambda r: r.arrival)
         # N_tri caches are computed (or read) before the clock starts
         for key in sorted({(r.Ngrid, r.Nmax, r.Ncut, r.step) for r in reqs}):
-            P._counts_Bk123(Ngrid=key[0], Nmax=key[1], Ncut=key[2], step=key[3], device=self.device)
+            try:
+                P._counts_Bk123(Ngrid=key[0], Nmax=key[1], Ncut=key[2], step=key[3],
+                                device=self.device)
+            except (L.PsxError, ValueError):
+                pass  # the request fails on its own and records the error
         if self.use_graphs:
             # one captured B(k) graph per slot and box shape, built before the
             # clock; the budget then covers what is left
@@ -170,7 +175,10 @@ class Server:
                             key=lambda k: -k[0])
             for slot in self.slots:
                 for key in shapes:
-                    slot.add_graph(key)
+                    try:
+                        slot.add_graph(key)
+                    except (L.PsxError, ValueError):
+                        pass  # unsupported shape: its requests fail and report it
             torch.cuda.synchronize(self.device)
             self.graph_build_s += time.perf_counter() - tg
             if not self._fixed_budget:
@@ -180,11 +188,15 @@ class Server:
         t0 = time.perf_counter()
 
         def work(req, s, need):
+            # a failing request (bad input, out of memory) records its error
+            # and frees its slot; the others are unaffected
             try:
                 req.t_start = time.perf_counter() - t0
                 req.result = self.slots[s].run(req)
-                req.t_done = time.perf_counter() - t0
+            except Exception as e:
+                req.error = "%s: %s" % (type(e).__name__, e)
             finally:
+                req.t_done = time.perf_counter() - t0
                 self._release(s, need)
 
         for req in reqs:
@@ -203,7 +215,11 @@ class Server:
 
 
 def latency_stats(reqs):
-    lat = np.array([r.t_done - r.arrival for r in reqs]) * 1e3
+    """Latency percentiles (ms) of the requests that completed."""
+    lat = np.array([r.t_done - r.arrival for r in reqs if r.error is None]) * 1e3
+    if lat.size == 0:
+        return {"p50_ms": float("nan"), "p99_ms": float("nan"), "mean_ms": float("nan"),
+                "max_ms": float("nan")}
     return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
 

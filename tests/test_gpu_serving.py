@@ -58,3 +58,24 @@ def test_admission_respects_budget():
     iv = sorted((r.t_start, r.t_done) for r in out)
     for (s0, e0), (s1, e1) in zip(iv, iv[1:]):
         assert s1 >= e0 - 1e-3
+
+
+def test_failed_request_is_isolated():
+    # an unsupported grid (62 = 2 x 31) fails with the library's error; the
+    # other requests complete and equal their results run alone
+    _say("serve failure isolation")
+    reqs = _shrink(serve.make_stream(4, rate=0.0, seed=10, grids=(48,), boxes=(500.,),
+                                     nrange=(2e4, 5e4), mesh=64))
+    reqs[1].Ngrid = 62
+    out, _ = serve.Server(nslots=2).serve(reqs)
+    bad = [r for r in out if r.rid == reqs[1].rid][0]
+    assert bad.result is None and bad.error is not None and "Ngrid must be" in bad.error
+    good = [r for r in out if r.error is None]
+    assert len(good) == 3 and all(r.result is not None for r in good)
+    alone = serve.Server(nslots=1)
+    for r in good:
+        ref = serve.Request(r.rid, r.xyz, r.Lbox, r.Ngrid, Nmax=r.Nmax, precision=r.precision)
+        o, _ = alone.serve([ref])
+        assert np.array_equal(o[0].result["b_counts"], r.result["b_counts"])
+    st = serve.latency_stats(out)
+    assert np.isfinite(st["p50_ms"])
