@@ -305,6 +305,45 @@ static void triangle_cover(const std::vector<int>& ijl, int nmax, int j0, std::v
 int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
                       const ContractPlan** out);
 
+// Split-BF16 contraction: the LDS serves the 16-byte A reads of 8 rows of a
+// pair tile per cycle, conflict-free when their distinct shells differ
+// modulo 8 (float rows 8 words apart modulo 64 banks).  Swap rows between
+// the two 8-row halves of each tile while that lowers the number of shells
+// that share a bank group with another shell of their half (i and j
+// columns).  Row order inside a tile does not change any result.
+static void arrange_half_tiles(std::vector<int>* prow, int nsh, int j0) {
+  auto half_cost = [&](const int* r) {
+    int c = 0;
+    for (int col = 0; col < 2; ++col) {
+      int seen[64], nseen = 0, grp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int k = 0; k < 8; ++k) {
+        if (r[k] < 0) continue;
+        const int sh = (col == 0 ? r[k] / nsh : r[k] % nsh) - j0;
+        bool dup = false;
+        for (int m = 0; m < nseen; ++m) dup |= seen[m] == sh;
+        if (dup) continue;
+        seen[nseen++] = sh;
+        c += grp[sh & 7]++ > 0 ? 1 : 0;
+      }
+    }
+    return c;
+  };
+  for (size_t t = 0; t + 16 <= prow->size(); t += 16) {
+    int* r = prow->data() + t;
+    int cur = half_cost(r) + half_cost(r + 8);
+    for (bool improved = true; improved;) {
+      improved = false;
+      for (int a = 0; a < 8; ++a)
+        for (int b = 8; b < 16; ++b) {
+          std::swap(r[a], r[b]);
+          const int c2 = half_cost(r) + half_cost(r + 8);
+          if (c2 < cur) { cur = c2; improved = true; }
+          else std::swap(r[a], r[b]);
+        }
+    }
+  }
+}
+
 int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
                       const ContractPlan** out) {
   const int cprec = prec;
@@ -345,6 +384,7 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
   cp->npt = (int)cdiv(cp->npairs, 16);
   prow.resize((size_t)cp->npt * 16, -1);
   const int nsh = nmax + 1;
+  if (prec == PSX_PREC_BF16X3) arrange_half_tiles(&prow, nsh, j0);
   std::vector<int> pair_s(2 * prow.size(), -1);
   std::map<int, int> prow_of;
   for (size_t r = 0; r < prow.size(); ++r) {

@@ -24,9 +24,19 @@
 //
 // K order.  The sum over grid points is order-free, so each mode maps the
 // MFMA k index to points in the way that gives 16-byte LDS reads: BF16X3
-// lane group kq holds 8 consecutive points of a 32-point step; F32/F64 lane
-// group kq owns the quarter [kq*KC/4, (kq+1)*KC/4) of the chunk and walks it
-// one point per MFMA, reading 4 points (4 MFMAs) per vector load.
+// lane group kq holds points 4kq..4kq+3 and 16+4kq..16+4kq+3 of a 32-point
+// step (two 16-byte reads of a float row; the hi/lo planes store each
+// 32-point step in that order, so the B fragment is one 16-byte read);
+// F32/F64 lane group kq owns the quarter [kq*KC/4, (kq+1)*KC/4) of the chunk
+// and walks it one point per MFMA, reading 4 points (4 MFMAs) per vector load.
+//
+// LDS banks (measured on gfx950): a ds_read_b128 serves 8 lanes of two lane
+// groups (16 lanes, 16 x 16 B = all 64 banks) per cycle, so it is
+// conflict-free when, among those, the 16-byte units (row base + column)
+// differ modulo 16.  Rows whose stride is 8 words modulo 64 banks and lane
+// groups 4 words apart achieve that for 8 distinct rows (modulo 8); rows 4
+// words apart with lane groups 8 words apart run at half
+// the LDS bandwidth.
 #include <algorithm>
 
 #include "psx_ctx.h"
@@ -70,10 +80,10 @@ template <> struct CtTraits<PSX_PREC_F32> {
   static constexpr int KC = 128, MAXPT = 2, PAD = 4;
 };
 #ifndef PSX_CT_PAD_BF
-#define PSX_CT_PAD_BF 4
+#define PSX_CT_PAD_BF 8     // float rows: 264 words = 8 modulo 64 banks
 #endif
 #ifndef PSX_CT_PADH_BF
-#define PSX_CT_PADH_BF 8
+#define PSX_CT_PADH_BF 16   // bf16 planes: 272 halves = 136 words = 8 modulo 64
 #endif
 template <> struct CtTraits<PSX_PREC_BF16X3> {
   typedef float TD; typedef f4v Acc; typedef double TP;
@@ -92,7 +102,7 @@ template <int MODE, int NLT> __host__ __device__ constexpr int ct_maxpt() {
   return (MODE == PSX_PREC_BF16X3 && NLT <= 3) ? PSX_CT_MAXPT_BF : CtTraits<MODE>::MAXPT;
 }
 
-// Row strides (elements): padded so that 8 consecutive rows start in
+// Row strides (elements): padded so that consecutive rows start in
 // different 16-byte bank groups and every row start stays 16-byte aligned.
 template <int MODE> __host__ __device__ constexpr int ct_ldw() {
   return CtTraits<MODE>::KC + CtTraits<MODE>::PAD;
@@ -200,7 +210,8 @@ __global__ __launch_bounds__((ct_waves<MODE, NLT>() * 64), PSX_CT_MINW) void k_c
   const int r16 = lane & 15;   // row (A) / column (B) within a 16-block
   const int kq = lane >> 4;    // k group
   const int gw = (int)blockIdx.y * CT_WAVES + wave;
-  const int kbase = BF ? kq * 8 : kq * KQ;
+  const int kbase = BF ? kq * 4 : kq * KQ;   // float rows
+  const int kbaseh = kq * 8;                  // bf16 planes (split-BF16)
   // bf16 pairs (lo, hi) = (-1, 0) and (0, -1) for the split residuals
   const unsigned neg_lo = in_register(0x0000BF80u), neg_hi = in_register(0xBF800000u);
 
@@ -222,7 +233,7 @@ __global__ __launch_bounds__((ct_waves<MODE, NLT>() * 64), PSX_CT_MINW) void k_c
   for (int t = 0; t < NLT; ++t) {
     const int s = t * 16 + r16;
     ob[t] = ((s < ns) ? s : ns) * LDW + kbase;
-    obh[t] = ((s < ns) ? s : ns) * LDWH + kbase;
+    obh[t] = ((s < ns) ? s : ns) * LDWH + kbaseh;
   }
   // F32/BF16X3: the MFMA accumulators restart from zero every chunk and are
   // added into float64 accumulators (acc2) at the end of the chunk, so no
@@ -305,8 +316,10 @@ __global__ __launch_bounds__((ct_waves<MODE, NLT>() * 64), PSX_CT_MINW) void k_c
             h[p] = hb;
             l[p] = pack_bf16(r0, r1);
           }
-          *reinterpret_cast<u2v*>(ldh + s * LDWH + k) = h;
-          *reinterpret_cast<u2v*>(ldl + s * LDWH + k) = l;
+          // point k = 32m + 16h + 4g is stored at 32m + 8g + 4h (the K order)
+          const int kp = (k & ~31) + ((k & 12) << 1) + ((k & 16) >> 2);
+          *reinterpret_cast<u2v*>(ldh + s * LDWH + kp) = h;
+          *reinterpret_cast<u2v*>(ldl + s * LDWH + kp) = l;
         }
       }
     }
@@ -331,7 +344,7 @@ __global__ __launch_bounds__((ct_waves<MODE, NLT>() * 64), PSX_CT_MINW) void k_c
           const f4v* pa = reinterpret_cast<const f4v*>(lds + oi[q] + k0);
           const f4v* pb = reinterpret_cast<const f4v*>(lds + oj[q] + k0);
           bf8v ah, al;
-          product_split_bf16(pa[0], pa[1], pb[0], pb[1], neg_lo, neg_hi, ah, al);
+          product_split_bf16(pa[0], pa[4], pb[0], pb[4], neg_lo, neg_hi, ah, al);
 #pragma unroll
           for (int t = 0; t < NLT; ++t) {
             if ((msk[q] >> t) & 1) {
