@@ -91,7 +91,7 @@ template <> struct CtTraits<PSX_PREC_BF16X3> {
 };
 
 // Waves per workgroup and pair tiles per wave.  Split-BF16 up to 3 l tiles:
-// 12 waves x 2 tiles (152 VGPRs, so 3 waves per SIMD overlap each other's LDS,
+// 12 waves x 2 tiles (168 VGPRs, so 3 waves per SIMD overlap each other's LDS,
 // VALU and MFMA latencies; measured 4.5 % faster than 8 x 4 at Nmax 40).
 // Otherwise 8 waves with the mode's tiles per wave (split-BF16 at 4 l tiles:
 // 3, whose float32 and float64 accumulators fit 256 VGPRs at 2 waves per SIMD).
