@@ -68,6 +68,9 @@ typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 #ifndef PSX_CT_KC_BF
 #define PSX_CT_KC_BF 256
 #endif
+#ifndef PSX_CT_KC_BF4
+#define PSX_CT_KC_BF4 128
+#endif
 constexpr int CT_MAXLT = 4;   // l tiles in the plan layout (ns <= 64)
 
 template <int MODE> struct CtTraits;
@@ -104,17 +107,24 @@ template <int MODE, int NLT> __host__ __device__ constexpr int ct_maxpt() {
 
 // Row strides (elements): padded so that consecutive rows start in
 // different 16-byte bank groups and every row start stays 16-byte aligned.
-template <int MODE> __host__ __device__ constexpr int ct_ldw() {
-  return CtTraits<MODE>::KC + CtTraits<MODE>::PAD;
+// Points per chunk.  Split-BF16 at 4 l tiles: PSX_CT_KC_BF4 = 128.  Its 8
+// waves x 3 tiles fill the 256 VGPRs of 2 waves per SIMD; with 256-point
+// chunks the prefetch registers spilled to scratch (C4: 45.8 ms), with 128
+// they fit (36.7 ms).
+template <int MODE, int NLT> __host__ __device__ constexpr int ct_kc() {
+  return (MODE == PSX_PREC_BF16X3 && NLT > 3) ? PSX_CT_KC_BF4 : CtTraits<MODE>::KC;
 }
-template <int MODE> __host__ __device__ constexpr int ct_ldwh() {
-  return CtTraits<MODE>::KC + PSX_CT_PADH_BF;
+template <int MODE, int NLT> __host__ __device__ constexpr int ct_ldw() {
+  return ct_kc<MODE, NLT>() + CtTraits<MODE>::PAD;
+}
+template <int MODE, int NLT> __host__ __device__ constexpr int ct_ldwh() {
+  return ct_kc<MODE, NLT>() + PSX_CT_PADH_BF;
 }
 
-template <int MODE>
+template <int MODE, int NLT>
 static size_t ct_lds_bytes(int ns) {
-  size_t b = (size_t)(ns + 1) * ct_ldw<MODE>() * sizeof(typename CtTraits<MODE>::TD);
-  if (MODE == PSX_PREC_BF16X3) b += (size_t)2 * (ns + 1) * ct_ldwh<MODE>() * sizeof(unsigned short);
+  size_t b = (size_t)(ns + 1) * ct_ldw<MODE, NLT>() * sizeof(typename CtTraits<MODE>::TD);
+  if (MODE == PSX_PREC_BF16X3) b += (size_t)2 * (ns + 1) * ct_ldwh<MODE, NLT>() * sizeof(unsigned short);
   return b;
 }
 
@@ -187,10 +197,10 @@ __global__ __launch_bounds__((ct_waves<MODE, NLT>() * 64), PSX_CT_MINW) void k_c
   typedef CtTraits<MODE> Tr;
   typedef typename Tr::TD TD;
   typedef typename Tr::Acc Acc;
-  constexpr int KC = Tr::KC, MAXPT = ct_maxpt<MODE, NLT>();
+  constexpr int KC = ct_kc<MODE, NLT>(), MAXPT = ct_maxpt<MODE, NLT>();
   constexpr int CT_WAVES = ct_waves<MODE, NLT>(), CT_THREADS = CT_WAVES * 64;
-  constexpr int LDW = ct_ldw<MODE>();
-  constexpr int LDWH = ct_ldwh<MODE>();
+  constexpr int LDW = ct_ldw<MODE, NLT>();
+  constexpr int LDWH = ct_ldwh<MODE, NLT>();
   constexpr bool BF = MODE == PSX_PREC_BF16X3;
   constexpr int KQ = KC / 4;  // F32/F64: points per lane group per chunk
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -497,14 +507,15 @@ static int contract_t(psx_ctx* ctx, const ContractPlan* cp, const void* D, int N
   constexpr int CT_WAVES = ct_waves<MODE, NLT>();
   PSX_REQUIRE(cp->waves % CT_WAVES == 0, "contraction plan built for another wave layout");
   const int ngroups = cp->waves / CT_WAVES;
-  const size_t lds = ct_lds_bytes<MODE>(cp->ns);
+  const size_t lds = ct_lds_bytes<MODE, NLT>(cp->ns);
   PSX_TRY(set_lds(k_contract<MODE, NLT>, lds));
   int slots = 0;
   PSX_TRY((contract_slots<MODE, NLT>(ctx, lds, &slots)));
-  const int64_t nchunks = cdiv(G, Tr::KC);
+  constexpr int KC = ct_kc<MODE, NLT>();
+  const int64_t nchunks = cdiv(G, KC);
   int64_t nwg = std::max<int64_t>(1, slots / ngroups);
   if (nwg > nchunks) nwg = nchunks;
-  const int64_t pts_per_block = cdiv(nchunks, nwg) * Tr::KC;
+  const int64_t pts_per_block = cdiv(nchunks, nwg) * KC;
   nwg = cdiv(G, pts_per_block);
   const int nslot = cp->nslot;
   void* part = nullptr;
