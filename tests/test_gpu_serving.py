@@ -1,7 +1,8 @@
 """GPU tests of the continuous-batching request loop (pyspectrum_amd/serve.py):
 requests overlap on the device, and each result equals the result of the
-same request run alone (B(k) bit for bit -- every B stage is deterministic --
-and P(k) to float64 round-off of its atomic bin sums)."""
+same request run alone (triangle counts bit for bit -- the shell and
+contraction stages are deterministic -- and every value that contains binned
+power, whose float64 atomic bin sums are order-dependent, to round-off)."""
 import numpy as np
 import pytest
 import torch
