@@ -125,8 +125,9 @@ int psx_hermitian_half(psx_ctx* ctx, const void* full, int in_prec, int64_t sx, 
 
 /* Replaces the binning loops of Pk_periodic (pyspectrum.py:691-714) and the
  * shell power of _Bk_periodic (:404, in Parseval form): over the full field
- * reflect_delta would build, bins b = int(|k|/step + 0.5) for b = 1..nbins;
- * out3 = [3][nbins] float64 sums of (kf*|k|, |delta|^2, 1). */
+ * reflect_delta would build, bins b = int(|k|/step + 0.5) for b = 1..nbins
+ * (1 <= nbins <= 1024); out3 = [3][nbins] float64 sums of (kf*|k|,
+ * |delta|^2, 1), summed in a fixed order (bit-identical run to run). */
 int psx_bin_power(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins, double kf,
                   double* out3, int prec);
 

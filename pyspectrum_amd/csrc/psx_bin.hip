@@ -11,36 +11,89 @@ namespace psx {
 // x = 0 and x = N/2 once, the seven Nyquist corners with Re(delta)^2.
 // bin = shell_of_m(|k|^2, step); bins 1..nbins are accumulated (k = kf*|k|).
 // Output sums (k, p, n) per bin; the caller divides.
+//
+// Deterministic: every sum is formed in a fixed order, so the result is
+// bit-identical run to run.  A wave takes 64 consecutive x of one (y, z) row
+// of the half field; |k|^2 = x^2 + c grows with x, so the lanes of one bin are
+// one contiguous run.  A segmented scan over the lanes (fixed shift order)
+// sums each run, and the run's last lane adds it to the wave's own
+// accumulators in LDS (one lane per bin: no two lanes share an address).
+// The block then adds its waves in order and stores a partial per block;
+// k_bin_reduce adds the blocks in a fixed order.
+constexpr int BP_THREADS = 256, BP_WAVES = BP_THREADS / 64, BP_MAX_BLOCKS = 512;
+constexpr int BP_MAX_BINS = 1024;  // LDS: BP_WAVES x 3 x nbins doubles
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bin_power(const cplx<T>* __restrict__ Hh, int N, int step,
-                                                   int nbins, double kf, double* __restrict__ out) {
+__global__ __launch_bounds__(BP_THREADS) void k_bin_power(const cplx<T>* __restrict__ Hh, int N,
+                                                          int step, int nbins, double kf,
+                                                          double* __restrict__ part) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* acc = reinterpret_cast<double*>(smem_raw);  // [3][nbins]
-  for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x) acc[i] = 0.0;
+  double* accw = reinterpret_cast<double*>(smem_raw);  // [BP_WAVES][3][nbins]
+  for (int i = threadIdx.x; i < BP_WAVES * 3 * nbins; i += BP_THREADS) accw[i] = 0.0;
   __syncthreads();
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  double* acc = accw + wave * 3 * nbins;
   const int Nh = N / 2 + 1, hg = N / 2;
-  const int64_t n = (int64_t)Nh * N * N;
-  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
-       id += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(id % Nh);
-    const int y = (int)((id / Nh) % N);
-    const int z = (int)(id / ((int64_t)Nh * N));
-    const int ay = min(y, N - y), az = min(z, N - z);
-    const int64_t m = (int64_t)x * x + (int64_t)ay * ay + (int64_t)az * az;
-    const int b = shell_of_m(m, step);
-    if (b < 1 || b > nbins) continue;
-    const cplx<T> v = Hh[id];
-    const bool xe = (x == 0 || x == hg);
-    const bool corner = xe && (y == 0 || y == hg) && (z == 0 || z == hg);
-    const double re = (double)v.re, im = corner ? 0.0 : (double)v.im;
-    const double w = xe ? 1.0 : 2.0;
-    atomicAdd(&acc[b - 1], w * (kf * sqrt((double)m)));
-    atomicAdd(&acc[nbins + b - 1], w * (re * re + im * im));
-    atomicAdd(&acc[2 * nbins + b - 1], w);
+  const int cpr = (Nh + 63) / 64;                     // 64-lane chunks per row
+  const int64_t items = (int64_t)N * N * cpr;
+  for (int64_t it = (int64_t)blockIdx.x * BP_WAVES + wave; it < items;
+       it += (int64_t)gridDim.x * BP_WAVES) {
+    const int64_t row = it / cpr;                     // row = z * N + y
+    const int x = (int)(it - row * cpr) * 64 + lane;
+    const int y = (int)(row % N), z = (int)(row / N);
+    int key = -1;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (x < Nh) {
+      const int ay = min(y, N - y), az = min(z, N - z);
+      const int64_t m = (int64_t)x * x + (int64_t)ay * ay + (int64_t)az * az;
+      const int b = shell_of_m(m, step);
+      if (b >= 1 && b <= nbins) {
+        const cplx<T> v = Hh[row * Nh + x];
+        const bool xe = (x == 0 || x == hg);
+        const bool corner = xe && (y == 0 || y == hg) && (z == 0 || z == hg);
+        const double re = (double)v.re, im = corner ? 0.0 : (double)v.im;
+        const double w = xe ? 1.0 : 2.0;
+        key = b - 1;
+        v0 = w * (kf * sqrt((double)m));
+        v1 = w * (re * re + im * im);
+        v2 = w;
+      }
+    }
+    // keys -1 (outside 1..nbins or x >= Nh) only lead or trail the valid lanes
+    if (__ballot(key >= 0) == 0) continue;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int ku = __shfl_up(key, off, 64);
+      const double u0 = __shfl_up(v0, off, 64), u1 = __shfl_up(v1, off, 64),
+                   u2 = __shfl_up(v2, off, 64);
+      if (lane >= off && ku == key) { v0 += u0; v1 += u1; v2 += u2; }
+    }
+    const int kn = __shfl_down(key, 1, 64);
+    if (key >= 0 && (lane == 63 || kn != key)) {
+      acc[key] += v0;
+      acc[nbins + key] += v1;
+      acc[2 * nbins + key] += v2;
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x)
-    if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
+  for (int i = threadIdx.x; i < 3 * nbins; i += BP_THREADS) {
+    double s = accw[i];
+#pragma unroll
+    for (int w = 1; w < BP_WAVES; ++w) s += accw[w * 3 * nbins + i];
+    part[(int64_t)blockIdx.x * 3 * nbins + i] = s;
+  }
+}
+
+// out[e] = sum over blocks of part[block][e]: lane l adds blocks l, l+64, ...
+// in order, then a fixed butterfly over the lanes.
+__global__ __launch_bounds__(64) void k_bin_reduce(const double* __restrict__ part, int nblocks,
+                                                   int nent, double* __restrict__ out) {
+  const int e = blockIdx.x, lane = threadIdx.x;
+  double s = 0.0;
+  for (int b = lane; b < nblocks; b += 64) s += part[(int64_t)b * nent + e];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[e] = s;
 }
 
 __global__ void k_fill_f64(double* p, int64_t n) {
@@ -51,21 +104,25 @@ __global__ void k_fill_f64(double* p, int64_t n) {
 
 int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
                      int prec, double* out3) {
-  PSX_REQUIRE(nbins >= 1 && nbins <= 4096, "nbins out of range");
-  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, ctx->stream, out3, (int64_t)3 * nbins);
-  PSX_CHECK_LAUNCH();
-  const size_t lds = (size_t)3 * nbins * sizeof(double);
-  const int64_t n = (int64_t)(N / 2 + 1) * N * N;
-  const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), 2048);
+  PSX_REQUIRE(nbins >= 1 && nbins <= BP_MAX_BINS, "nbins must be in [1, 1024]");
+  const size_t lds = (size_t)BP_WAVES * 3 * nbins * sizeof(double);
+  const int Nh = N / 2 + 1;
+  const int64_t items = (int64_t)N * N * ((Nh + 63) / 64);
+  const int blocks = (int)std::min<int64_t>(cdiv(items, BP_WAVES), BP_MAX_BLOCKS);
+  void* part = nullptr;
+  PSX_TRY(ctx_buffer(ctx, "bin_partials", (size_t)blocks * 3 * nbins * sizeof(double), &part));
   if (prec == PSX_PREC_F64) {
-    if (lds > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_bin_power<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_bin_power<double>, dim3(blocks), dim3(256), lds, ctx->stream,
-                       (const cplx<double>*)half, N, step, nbins, kf, out3);
+    PSX_TRY(set_lds(k_bin_power<double>, lds));
+    hipLaunchKernelGGL(k_bin_power<double>, dim3(blocks), dim3(BP_THREADS), lds, ctx->stream,
+                       (const cplx<double>*)half, N, step, nbins, kf, (double*)part);
   } else {
-    if (lds > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_bin_power<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_bin_power<float>, dim3(blocks), dim3(256), lds, ctx->stream,
-                       (const cplx<float>*)half, N, step, nbins, kf, out3);
+    PSX_TRY(set_lds(k_bin_power<float>, lds));
+    hipLaunchKernelGGL(k_bin_power<float>, dim3(blocks), dim3(BP_THREADS), lds, ctx->stream,
+                       (const cplx<float>*)half, N, step, nbins, kf, (double*)part);
   }
+  PSX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_bin_reduce, dim3(3 * nbins), dim3(64), 0, ctx->stream, (const double*)part,
+                     blocks, 3 * nbins, out3);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }

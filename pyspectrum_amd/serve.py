@@ -11,11 +11,9 @@ and B(k) (Bk_periodic, pyspectrum.py:285).  The loop
 * computes the half field once per request and reuses it for P_l and B;
 * records arrival, start and completion times (latency = completion - arrival).
 
-Every stage is deterministic except the float64 atomics of the power
-binning (P_l and the shell powers p0k), so a request's triangle sums B_raw
-and counts are bit-identical whether it ran alone or in the mix, and every
-value that contains binned power (P_l, p0k, and b123/q123 through their
-shot-noise terms) agrees to float64 round-off.
+Every stage is deterministic except the float64 atomics of the P_l binning
+(pk_pbox_rsd), so a request's B(k) outputs are bit-identical whether it ran
+alone or in the mix, and its P_l agrees to float64 round-off.
 """
 import ctypes
 import math

@@ -227,9 +227,28 @@ def test_bk_raw_is_deterministic():
     outs = []
     for _ in range(2):
         braw = torch.empty(150, dtype=torch.float64, device=DEV)
-        L.call("psx_bk_raw", _h(), L.ptr(half), N, 3, 3, 10, L.ptr(braw), None, L.PREC["fp32"])
-        outs.append(braw.cpu().numpy())
-    assert np.array_equal(outs[0], outs[1])
+        p0k = torch.empty(10, dtype=torch.float64, device=DEV)
+        L.call("psx_bk_raw", _h(), L.ptr(half), N, 3, 3, 10, L.ptr(braw), L.ptr(p0k), L.PREC["fp32"])
+        outs.append((braw.cpu().numpy(), p0k.cpu().numpy()))
+    # triangle sums and the shell powers (fixed-order binning) bit for bit
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_binned_power_is_deterministic(prec):
+    # Pk_periodic and the full B(k) output repeat bit for bit (every sum in a
+    # fixed order), at a grid with several 64-lane chunks per row
+    _say("binned power determinism", prec)
+    N, Lbox = 144, 1000.0
+    xyz, w = _particles(40000, Lbox, 23, weights=True)
+    pk = [P.Pk_periodic(xyz, w=w, Lbox=Lbox, Ngrid=N, precision=prec) for _ in range(2)]
+    for k in ("k", "p0k", "counts"):
+        assert np.array_equal(pk[0][k], pk[1][k]), k
+    bk = [P.Bk_periodic(xyz, w=w, Lbox=Lbox, Ngrid=N, step=3, Ncut=3, Nmax=12, precision=prec)
+          for _ in range(2)]
+    for k in ("p0k1", "p0k2", "p0k3", "b123", "q123", "b123_sn", "counts"):
+        assert np.array_equal(bk[0][k], bk[1][k]), k
 
 
 # --------------------------------------------------------------- N_tri -----

@@ -1,8 +1,8 @@
 """GPU tests of the continuous-batching request loop (pyspectrum_amd/serve.py):
 requests overlap on the device, and each result equals the result of the
-same request run alone (triangle counts bit for bit -- the shell and
-contraction stages are deterministic -- and every value that contains binned
-power, whose float64 atomic bin sums are order-dependent, to round-off)."""
+same request run alone (every B(k) output bit for bit -- its stages sum in
+a fixed order -- and P_l, whose pk_pbox_rsd bins are float64 atomic sums, to
+round-off)."""
 import numpy as np
 import pytest
 import torch
@@ -39,11 +39,11 @@ def test_results_identical_alone_and_in_mix():
         ref = serve.Request(r.rid, r.xyz, r.Lbox, r.Ngrid, Nmax=r.Nmax, precision=r.precision)
         out, _ = alone_server.serve([ref])
         a, b = out[0].result, r.result
-        # triangle sums and counts are bit-identical; P(k) bins sum with float64
-        # atomics, so quantities that subtract shot noise agree to round-off
-        assert np.array_equal(a["b_counts"], b["b_counts"]), r.rid
-        assert np.array_equal(a["counts"], b["counts"]), r.rid
-        for key in ("b123", "q123", "p0k1", "b123_sn", "p0k", "p2k", "k"):
+        # every B(k) output is bit-identical; the P_l bins sum with float64
+        # atomics, so P_l agrees to round-off
+        for key in ("b_counts", "b123", "q123", "p0k1", "p0k2", "p0k3", "b123_sn", "counts"):
+            assert np.array_equal(a[key], b[key]), (r.rid, key)
+        for key in ("p0k", "p2k", "k"):
             den = np.maximum(np.abs(b[key]), np.max(np.abs(b[key])) * 1e-6)
             assert np.max(np.abs(a[key] - b[key]) / den) < 1e-10, (r.rid, key)
 
