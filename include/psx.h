@@ -112,6 +112,35 @@ int psx_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, c
 int psx_fft_survey_mono(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                         double lbox, double wsum_abs, void* half_out, int prec);
 
+/* Quadrupole machinery (the reference's unfinished survey quadrupole,
+ * pyspectrum.py:828-900 and :1083-1131, whose Python calls cannot run;
+ * these replace the Fortran routines they call).  Half fields as everywhere
+ * here: complex [N][N][N/2+1] (x fastest = the Fortran (N/2+1, N, N)).
+ * The float path replays the Fortran's single-precision operations in order;
+ * modes with |k| = 0 are left unchanged.
+ *
+ * psx_quad_weights replaces the weight branch of assign_quad
+ * (estimator.f:291-299): w_out = w (ia..id all 0), w r_ia r_ib / |r|^2
+ * (ic = id = 0) or w r_ia r_ib r_ic r_id / |r|^4 (components 1..3); feed
+ * w_out to psx_assign_quad / psx_fft_periodic for the Q_ij fields.  w may
+ * be NULL (unit weights). */
+int psx_quad_weights(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ia, int ib,
+                     int ic, int id, float* w_out);
+/* estimator.f:514-540 FiveDelta2g_1: dcgxx = 7.5 (dcgxx kxh^2 + dcgyy kyh^2
+ * + dcgzz kzh^2).  unit_vectors = 0 reproduces the Fortran, whose undeclared
+ * kxh/kyh/kzh are INTEGER (float(k)/rk truncated: +-1 on the axes, else 0);
+ * unit_vectors = 1 uses the real unit vectors k_i/|k|. */
+int psx_five_delta2g_1(psx_ctx* ctx, void* dcgxx, const void* dcgyy, const void* dcgzz, int ngrid,
+                       int unit_vectors, int prec);
+/* estimator.f:542-572 FiveDelta2g_2: dcgxx = dcgxx + 7.5 (2 dcgxy kxh kyh +
+ * 2 dcgyz kyh kzh + 2 dcgzx kzh kxh) - 2.5 dcg; unit_vectors as above. */
+int psx_five_delta2g_2(psx_ctx* ctx, const void* dcg, void* dcgxx, const void* dcgxy,
+                       const void* dcgyz, const void* dcgzx, int ngrid, int unit_vectors, int prec);
+/* estimator.f:574-604 build_quad: dclr2 = (7.5 mu^2 - 2.5) dclr1 with
+ * mu = k_los/|k|, line of sight irsd = 1, 2, 3 (x, y, z); the Fortran STOPs
+ * on any other irsd, this returns PSX_ERR_ARG. */
+int psx_build_quad(psx_ctx* ctx, const void* dclr1, void* dclr2, int irsd, int ngrid, int prec);
+
 /* Replaces reflect_delta (pyspectrum.py:1134-1157): half field -> full
  * field written C-order [x][y][z] (z fastest), as the reference returns it. */
 int psx_reflect_delta(psx_ctx* ctx, const void* half, void* full_out, int ngrid, int prec);

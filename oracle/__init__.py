@@ -13,12 +13,13 @@ Contents
               function cites the reference lines it follows.
 ``ref``       ctypes binding to ``oracle/_ref/libestimator.so``, the reference's
               own Fortran kernels (``assign_quad``, ``fcomb_periodic``,
-              ``fcomb_survey``, ``pk_pbox_rsd``) compiled from
+              ``fcomb_survey``, ``pk_pbox_rsd``, ``FiveDelta2g_1/2``,
+              ``build_quad``) compiled from
               ``/root/reference`` by ``oracle/Makefile``.  Used only to pin
               ``spectrum`` in tests.
 
-Pins (how each oracle function is checked; see ``tests/test_oracle.py`` and
-``tests/test_survey_oracle.py``):
+Pins (how each oracle function is checked; see ``tests/test_oracle.py``,
+``tests/test_survey_oracle.py`` and ``tests/test_quad_oracle.py``):
 
 * ``assign_quad``       -- the reference Fortran (``ref``), mass/normalisation
                            invariants, the 1-4-1 node stencil.
@@ -40,4 +41,8 @@ Pins (how each oracle function is checked; see ``tests/test_oracle.py`` and
 * ``radecz_to_cartesian`` -- axis points.
 * ``B0_survey_from_sums`` -- closed form when data = randoms (delta = 0).
 * ``FFT_survey_mono`` / ``B0_survey`` -- compositions of the pinned pieces.
+* ``five_delta2g_1/2``, ``build_quad`` -- the reference Fortran bit for bit,
+                           closed forms (factor 5 / -2.5, equal inputs).
+* ``quad_weights``      -- the reference Fortran ``assign_quad`` with
+                           (ia, ib, ic, id) != 0, traces sum to w.
 """

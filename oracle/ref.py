@@ -6,7 +6,8 @@ to pin the numpy restatement in ``oracle.spectrum`` and never by the product.
 
 Fortran calling convention: every argument by reference, symbol name with a
 trailing underscore, arrays column-major (numpy order='F').  The argument
-order is the Fortran one (estimator.f:155, :284, :605), not f2py's.
+order is the Fortran one (estimator.f:155, :284, :514, :542, :574, :605), not
+f2py's.
 """
 import ctypes
 import os
@@ -44,17 +45,46 @@ def _f(v):
     return ctypes.byref(ctypes.c_float(float(v)))
 
 
-def assign_quad(xyz32, w, Ngrid, kf_ks, offset=0.0):
-    """estimator.f:284 assign_quad(r, w, dtl, Np, Ngrid, kf_ks, offset, 0,0,0,0).
+def assign_quad(xyz32, w, Ngrid, kf_ks, offset=0.0, iabcd=(0, 0, 0, 0)):
+    """estimator.f:284 assign_quad(r, w, dtl, Np, Ngrid, kf_ks, offset, ia, ib, ic, id).
 
     Returns the interleaved float32 grid dtl(2*Ngrid, Ngrid, Ngrid) as an F-order
     array, plus its (A, B) views indexed [ix, iy, iz]."""
     r = np.asfortranarray(np.asarray(xyz32, dtype=np.float32))
     ww = np.asfortranarray(np.asarray(w, dtype=np.float32))
     dtl = np.zeros((2 * Ngrid, Ngrid, Ngrid), dtype=np.float32, order="F")
+    ia, ib, ic, id_ = iabcd
     lib().assign_quad_(_p(r), _p(ww), _p(dtl), _i(r.shape[1]), _i(Ngrid), _f(kf_ks), _f(offset),
-                       _i(0), _i(0), _i(0), _i(0))
+                       _i(ia), _i(ib), _i(ic), _i(id_))
     return dtl, dtl[0::2], dtl[1::2]
+
+
+def _half(F, Ngrid):
+    d = np.asfortranarray(np.asarray(F, dtype=np.complex64)).copy(order="F")
+    assert d.shape == (Ngrid // 2 + 1, Ngrid, Ngrid)
+    return d
+
+
+def five_delta2g_1(xx, yy, zz, Ngrid):
+    """estimator.f:514 FiveDelta2g_1(dcgxx, dcgyy, dcgzz, Ngrid); returns the new dcgxx."""
+    x, y, z = _half(xx, Ngrid), _half(yy, Ngrid), _half(zz, Ngrid)
+    lib().fivedelta2g_1_(_p(x), _p(y), _p(z), _i(Ngrid))
+    return x
+
+
+def five_delta2g_2(g, xx, xy, yz, zx, Ngrid):
+    """estimator.f:542 FiveDelta2g_2(dcg, dcgxx, dcgxy, dcgyz, dcgzx, Ngrid); returns dcgxx."""
+    a = [_half(v, Ngrid) for v in (g, xx, xy, yz, zx)]
+    lib().fivedelta2g_2_(*[_p(v) for v in a], _i(Ngrid))
+    return a[1]
+
+
+def build_quad(d1, d2, irsd, Ngrid):
+    """estimator.f:574 build_quad(dclr1, dclr2, irsd, Ngrid); returns the new dclr2."""
+    a, b = _half(d1, Ngrid), _half(d2, Ngrid)
+    assert irsd in (1, 2, 3)  # the Fortran STOPs otherwise
+    lib().build_quad_(_p(a), _p(b), _i(irsd), _i(Ngrid))
+    return b
 
 
 def fcomb_periodic(F, sumw, Ngrid):

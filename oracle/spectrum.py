@@ -683,3 +683,93 @@ def Pk_periodic_rsd(xyz, w=None, Lbox=2600, Ngrid=360, rsd=2, Nmubin=10):
             "k": k, "p0k": p0 - 1.0 / nbar, "p2k": p2, "p4k": p4,
             "p_sn": np.repeat(1.0 / nbar, len(k)), "counts": nk,
             "k_kmu": km, "mu_kmu": mk, "p_kmu": pkm - 1.0 / nbar, "counts_kmu": nkm}
+
+
+# ------------------------------------------------- quadrupole machinery -----
+# estimator.f:514-604 and the weight branch of assign_quad (:291-299).  Half
+# fields are indexed [ix, iy, iz] with ix = 0..N/2 (the Fortran (N/2+1, N, N)).
+# Float32 semantics as in the Fortran: rk = sqrt(float(m)), unit components
+# float(k)/rk, in FiveDelta2g_1/2 truncated to INTEGER (implicit typing of the
+# undeclared kxh, kyh, kzh) unless unit=True.  Complex values are combined
+# part by part in float32 in the Fortran's order; k = 0 is left unchanged.
+def _quad_modes(Ngrid):
+    n = np.arange(Ngrid)
+    f = np.where(n <= Ngrid // 2, n, n - Ngrid)
+    kx = np.arange(Ngrid // 2 + 1)[:, None, None]
+    ky = f[None, :, None]
+    kz = f[None, None, :]
+    m = (kx * kx + ky * ky + kz * kz).astype(np.int64)
+    rk = np.sqrt(m.astype(f32))
+    return kx, ky, kz, rk
+
+
+def _quad_units(Ngrid, unit):
+    kx, ky, kz, rk = _quad_modes(Ngrid)
+    ok = rk > 0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        u = [np.where(ok, (np.broadcast_to(k, rk.shape).astype(f32) / rk).astype(f32), f32(0))
+             for k in (kx, ky, kz)]
+    if not unit:
+        u = [np.trunc(v).astype(f32) for v in u]
+    return ok, u
+
+
+def five_delta2g_1(xx, yy, zz, Ngrid, unit=False):
+    """estimator.f:514-540: dcgxx = 7.5*(dcgxx*kxh**2 + dcgyy*kyh**2 + dcgzz*kzh**2)."""
+    ok, (ux, uy, uz) = _quad_units(Ngrid, unit)
+    kx2, ky2, kz2 = ux * ux, uy * uy, uz * uz
+    out = np.array(xx, dtype=np.complex64, copy=True)
+    parts = []
+    for part in (np.real, np.imag):
+        a, b, c = (part(np.asarray(v, np.complex64)).astype(f32) for v in (xx, yy, zz))
+        parts.append(f32(7.5) * ((a * kx2 + b * ky2) + c * kz2))
+    new = (parts[0] + 1j * parts[1]).astype(np.complex64)
+    out[ok] = new[ok]
+    return out
+
+
+def five_delta2g_2(g, xx, xy, yz, zx, Ngrid, unit=False):
+    """estimator.f:542-572: dcgxx = dcgxx + 7.5*(2*dcgxy*kxh*kyh + 2*dcgyz*kyh*kzh
+    + 2*dcgzx*kzh*kxh) - 2.5*dcg."""
+    ok, (ux, uy, uz) = _quad_units(Ngrid, unit)
+    out = np.array(xx, dtype=np.complex64, copy=True)
+    parts = []
+    for part in (np.real, np.imag):
+        d, a, p, q, r = (part(np.asarray(v, np.complex64)).astype(f32) for v in (g, xx, xy, yz, zx))
+        two = f32(2)
+        s = ((((two * p) * ux) * uy + ((two * q) * uy) * uz) + ((two * r) * uz) * ux)
+        parts.append((a + f32(7.5) * s) - f32(2.5) * d)
+    new = (parts[0] + 1j * parts[1]).astype(np.complex64)
+    out[ok] = new[ok]
+    return out
+
+
+def build_quad(d1, d2, irsd, Ngrid):
+    """estimator.f:574-604: dclr2 = (7.5*amu**2 - 2.5)*dclr1, amu = k_los/rk with
+    irsd 1, 2, 3 = x, y, z (the Fortran STOPs otherwise)."""
+    if irsd not in (1, 2, 3):
+        raise ValueError("irsd must be 1, 2 or 3")
+    kx, ky, kz, rk = _quad_modes(Ngrid)
+    kl = (kx, ky, kz)[irsd - 1]
+    ok = rk > 0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        amu = (np.broadcast_to(kl, rk.shape).astype(f32) / rk).astype(f32)
+    fac = f32(7.5) * (amu * amu) - f32(2.5)
+    a = np.asarray(d1, np.complex64)
+    new = ((fac * a.real.astype(f32)) + 1j * (fac * a.imag.astype(f32))).astype(np.complex64)
+    out = np.array(d2, dtype=np.complex64, copy=True)
+    out[ok] = new[ok]
+    return out
+
+
+def quad_weights(xyz32, w, ia, ib, ic, id_):
+    """estimator.f:291-299: w, w*r(ia)*r(ib)/rnorm or w*r(ia)*r(ib)*r(ic)*r(id)/rnorm**2
+    with rnorm = r(1)**2 + r(2)**2 + r(3)**2, all float32, left to right."""
+    r = np.asarray(xyz32, dtype=f32)
+    ww = np.ones(r.shape[1], f32) if w is None else np.asarray(w, dtype=f32)
+    if ia == 0 and ib == 0 and ic == 0 and id_ == 0:
+        return ww.copy()
+    rnorm = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]
+    if ic == 0 and id_ == 0:
+        return ((ww * r[ia - 1]) * r[ib - 1]) / rnorm
+    return ((((ww * r[ia - 1]) * r[ib - 1]) * r[ic - 1]) * r[id_ - 1]) / (rnorm * rnorm)

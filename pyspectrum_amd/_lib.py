@@ -37,6 +37,10 @@ SIGNATURES = {
     "psx_fft_survey_mono": [_p, _p, _p, _i64, _i, _d, _d, _p, _i],
     "psx_survey_catalogue": [_p, _p, _p, _p, _i64, _p, _p, _i, _d, _d, _d, _p, _p, _p],
     "psx_reflect_delta": [_p, _p, _p, _i, _i],
+    "psx_quad_weights": [_p, _p, _p, _i64, _i, _i, _i, _i, _p],
+    "psx_five_delta2g_1": [_p, _p, _p, _p, _i, _i, _i],
+    "psx_five_delta2g_2": [_p, _p, _p, _p, _p, _p, _i, _i, _i],
+    "psx_build_quad": [_p, _p, _p, _i, _i, _i],
     "psx_hermitian_half": [_p, _p, _i, _i64, _i64, _i64, _p, _i, _i],
     "psx_bin_power": [_p, _p, _i, _i, _i, _d, _p, _i],
     "psx_pk_pbox_rsd": [_p, _p, _i, _i, _i, _i, _i, _p, _i],

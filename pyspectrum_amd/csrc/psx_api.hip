@@ -639,6 +639,47 @@ int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np,
   return fft_periodic_impl(ctx, xyz, w, np, ngrid, lbox, sumw, wsum_abs, half_out, prec);
 }
 
+int psx_quad_weights(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ia, int ib,
+                     int ic, int id, float* w_out) {
+  PSX_REQUIRE(ctx && w_out && (xyz || np == 0), "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(np >= 0, "bad particle count");
+  const bool none = ia == 0 && ib == 0 && ic == 0 && id == 0;
+  const bool qij = ia >= 1 && ia <= 3 && ib >= 1 && ib <= 3 && ic == 0 && id == 0;
+  const bool qijkl = ia >= 1 && ia <= 3 && ib >= 1 && ib <= 3 && ic >= 1 && ic <= 3 && id >= 1 &&
+                     id <= 3;
+  PSX_REQUIRE(none || qij || qijkl, "ia, ib, ic, id: all 0, (i, j, 0, 0) or (i, j, k, l) in 1..3");
+  return launch_quad_weights(ctx, xyz, w, np, ia, ib, ic, id, w_out);
+}
+
+int psx_five_delta2g_1(psx_ctx* ctx, void* dcgxx, const void* dcgyy, const void* dcgzz, int ngrid,
+                       int unit_vectors, int prec) {
+  PSX_REQUIRE(ctx && dcgxx && dcgyy && dcgzz, "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(ngrid >= 2 && ngrid <= 4096, "Ngrid must be in [2, 4096]");
+  return launch_five_delta2_1(ctx, dcgxx, dcgyy, dcgzz, ngrid, unit_vectors ? 1 : 0, fft_prec(prec));
+}
+
+int psx_five_delta2g_2(psx_ctx* ctx, const void* dcg, void* dcgxx, const void* dcgxy,
+                       const void* dcgyz, const void* dcgzx, int ngrid, int unit_vectors, int prec) {
+  PSX_REQUIRE(ctx && dcg && dcgxx && dcgxy && dcgyz && dcgzx, "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(ngrid >= 2 && ngrid <= 4096, "Ngrid must be in [2, 4096]");
+  return launch_five_delta2_2(ctx, dcg, dcgxx, dcgxy, dcgyz, dcgzx, ngrid, unit_vectors ? 1 : 0,
+                              fft_prec(prec));
+}
+
+int psx_build_quad(psx_ctx* ctx, const void* dclr1, void* dclr2, int irsd, int ngrid, int prec) {
+  PSX_REQUIRE(ctx && dclr1 && dclr2, "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(irsd >= 1 && irsd <= 3, "irsd must be 1, 2 or 3 (line of sight x, y, z)");
+  PSX_REQUIRE(ngrid >= 2 && ngrid <= 4096, "Ngrid must be in [2, 4096]");
+  return launch_build_quad(ctx, dclr1, dclr2, irsd, ngrid, fft_prec(prec));
+}
+
 int psx_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, const double* w,
                          int64_t n, const double* dist_nodes, const double* dist_slopes, int nseg,
                          double zmax, double h, double p0_fkp, float* xyz_out, float* w_out,

@@ -175,4 +175,11 @@ int launch_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb
                             int64_t n, const double* dist_nodes, const double* dist_slopes, int nseg,
                             double zmax, double h, double p0, float* xyz, float* wout,
                             double* sums);
+int launch_five_delta2_1(psx_ctx* ctx, void* xx, const void* yy, const void* zz, int N, int unit,
+                         int fp);
+int launch_five_delta2_2(psx_ctx* ctx, const void* g, void* xx, const void* xy, const void* yz,
+                         const void* zx, int N, int unit, int fp);
+int launch_build_quad(psx_ctx* ctx, const void* d1, void* d2, int irsd, int N, int fp);
+int launch_quad_weights(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ia, int ib,
+                        int ic, int id, float* out);
 }  // namespace psx
