@@ -384,7 +384,7 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
   cp->npt = (int)cdiv(cp->npairs, 16);
   prow.resize((size_t)cp->npt * 16, -1);
   const int nsh = nmax + 1;
-  if (prec == PSX_PREC_BF16X3) arrange_half_tiles(&prow, nsh, j0);
+  if (prec != PSX_PREC_F64) arrange_half_tiles(&prow, nsh, j0);
   std::vector<int> pair_s(2 * prow.size(), -1);
   std::map<int, int> prow_of;
   for (size_t r = 0; r < prow.size(); ++r) {

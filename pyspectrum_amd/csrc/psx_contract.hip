@@ -59,6 +59,9 @@ typedef unsigned short us8v __attribute__((ext_vector_type(8)));
 #ifndef PSX_CT_WAVES_BF
 #define PSX_CT_WAVES_BF 12
 #endif
+#ifndef PSX_CT_WAVES_F32
+#define PSX_CT_WAVES_F32 12
+#endif
 #ifndef PSX_CT_MAXPT_BF
 #define PSX_CT_MAXPT_BF 2
 #endif
@@ -96,10 +99,15 @@ template <> struct CtTraits<PSX_PREC_BF16X3> {
 // Waves per workgroup and pair tiles per wave.  Split-BF16 up to 3 l tiles:
 // 12 waves x 2 tiles (168 VGPRs, so 3 waves per SIMD overlap each other's LDS,
 // VALU and MFMA latencies; measured 4.5 % faster than 8 x 4 at Nmax 40).
-// Otherwise 8 waves with the mode's tiles per wave (split-BF16 at 4 l tiles:
-// 3, whose float32 and float64 accumulators fit 256 VGPRs at 2 waves per SIMD).
+// FP32: 12 waves x 2 tiles (at most 166 VGPRs): 24 pair tiles per group, so
+// at 40 shells (22 tiles) one group reads D once (8 x 2 needed two groups,
+// each reading all of D).  Otherwise 8 waves with the mode's tiles per wave
+// (split-BF16 at 4 l tiles: 3, whose float32 and float64 accumulators fit
+// 256 VGPRs at 2 waves per SIMD).
 template <int MODE, int NLT> __host__ __device__ constexpr int ct_waves() {
-  return (MODE == PSX_PREC_BF16X3 && NLT <= 3) ? PSX_CT_WAVES_BF : PSX_CT_WAVES;
+  return (MODE == PSX_PREC_BF16X3 && NLT <= 3) ? PSX_CT_WAVES_BF
+         : MODE == PSX_PREC_F32                ? PSX_CT_WAVES_F32
+                                               : PSX_CT_WAVES;
 }
 template <int MODE, int NLT> __host__ __device__ constexpr int ct_maxpt() {
   return (MODE == PSX_PREC_BF16X3 && NLT <= 3) ? PSX_CT_MAXPT_BF : CtTraits<MODE>::MAXPT;
