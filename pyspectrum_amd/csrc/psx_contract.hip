@@ -81,9 +81,12 @@ template <> struct CtTraits<PSX_PREC_F64> {
   typedef double TD; typedef d4v Acc; typedef double TP;
   static constexpr int KC = 128, MAXPT = 4, PAD = 2;
 };
+#ifndef PSX_CT_KC_F32
+#define PSX_CT_KC_F32 128
+#endif
 template <> struct CtTraits<PSX_PREC_F32> {
   typedef float TD; typedef f4v Acc; typedef double TP;
-  static constexpr int KC = 128, MAXPT = 2, PAD = 4;
+  static constexpr int KC = PSX_CT_KC_F32, MAXPT = 2, PAD = 4;
 };
 #ifndef PSX_CT_PAD_BF
 #define PSX_CT_PAD_BF 8     // float rows: 264 words = 8 modulo 64 banks

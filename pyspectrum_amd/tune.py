@@ -38,6 +38,7 @@ CANDIDATES = {
     "ct_kc4_256": {"PSX_CT_KC_BF4": 256},
     "ct_kc4_64": {"PSX_CT_KC_BF4": 64},
     "ct_f32_w8": {"PSX_CT_WAVES_F32": 8},
+    "ct_f32_kc256": {"PSX_CT_KC_F32": 256},
     "shellx_p8": {"PSX_SHELL_X_PAIRS": 8},
     "shellx_p4": {"PSX_SHELL_X_PAIRS": 4},
     "fft_lines8": {"PSX_TILE_LINES_F32": 8},
