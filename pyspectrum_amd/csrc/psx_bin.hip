@@ -13,9 +13,10 @@ namespace psx {
 // Output sums (k, p, n) per bin; the caller divides.
 //
 // Deterministic: every sum is formed in a fixed order, so the result is
-// bit-identical run to run.  A wave takes 64 consecutive x of one (y, z) row
-// of the half field; |k|^2 = x^2 + c grows with x, so the lanes of one bin are
-// one contiguous run.  A segmented scan over the lanes (fixed shift order)
+// bit-identical run to run.  A wave takes whole (y, z) rows of the half field,
+// 64 consecutive x at a time; |k|^2 = x^2 + c grows with x, so the lanes of
+// one bin are one contiguous run (and a row or chunk that starts past the
+// last bin is skipped with everything after it).  A segmented scan over the lanes (fixed shift order)
 // sums each run, and the run's last lane adds it to the wave's own
 // accumulators in LDS (one lane per bin: no two lanes share an address).
 // The block then adds its waves in order and stores a partial per block;
@@ -34,45 +35,48 @@ __global__ __launch_bounds__(BP_THREADS) void k_bin_power(const cplx<T>* __restr
   const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   double* acc = accw + wave * 3 * nbins;
   const int Nh = N / 2 + 1, hg = N / 2;
-  const int cpr = (Nh + 63) / 64;                     // 64-lane chunks per row
-  const int64_t items = (int64_t)N * N * cpr;
-  for (int64_t it = (int64_t)blockIdx.x * BP_WAVES + wave; it < items;
-       it += (int64_t)gridDim.x * BP_WAVES) {
-    const int64_t row = it / cpr;                     // row = z * N + y
-    const int x = (int)(it - row * cpr) * 64 + lane;
-    const int y = (int)(row % N), z = (int)(row / N);
-    int key = -1;
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-    if (x < Nh) {
-      const int ay = min(y, N - y), az = min(z, N - z);
-      const int64_t m = (int64_t)x * x + (int64_t)ay * ay + (int64_t)az * az;
-      const int b = shell_of_m(m, step);
-      if (b >= 1 && b <= nbins) {
-        const cplx<T> v = Hh[row * Nh + x];
-        const bool xe = (x == 0 || x == hg);
-        const bool corner = xe && (y == 0 || y == hg) && (z == 0 || z == hg);
-        const double re = (double)v.re, im = corner ? 0.0 : (double)v.im;
-        const double w = xe ? 1.0 : 2.0;
-        key = b - 1;
-        v0 = w * (kf * sqrt((double)m));
-        v1 = w * (re * re + im * im);
-        v2 = w;
+  const int64_t mlast = shell_mmax(nbins, step);  // largest |k|^2 of bin nbins
+  // a wave takes whole (y, z) rows, 64 consecutive x at a time, in order
+  for (int row = (int)blockIdx.x * BP_WAVES + wave; row < N * N; row += (int)gridDim.x * BP_WAVES) {
+    const int y = row % N, z = row / N;
+    const int ay = min(y, N - y), az = min(z, N - z);
+    const int64_t c = (int64_t)ay * ay + (int64_t)az * az;
+    if (c > mlast) continue;                       // the whole row lies past bin nbins
+    for (int x0 = 0; x0 < Nh; x0 += 64) {
+      if (c + (int64_t)x0 * x0 > mlast) break;     // this and later chunks lie past it
+      const int x = x0 + lane;
+      int key = -1;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+      if (x < Nh) {
+        const int64_t m = c + (int64_t)x * x;
+        const int b = shell_of_m(m, step);
+        if (b >= 1 && b <= nbins) {
+          const cplx<T> v = Hh[(int64_t)row * Nh + x];
+          const bool xe = (x == 0 || x == hg);
+          const bool corner = xe && (y == 0 || y == hg) && (z == 0 || z == hg);
+          const double re = (double)v.re, im = corner ? 0.0 : (double)v.im;
+          const double w = xe ? 1.0 : 2.0;
+          key = b - 1;
+          v0 = w * (kf * sqrt((double)m));
+          v1 = w * (re * re + im * im);
+          v2 = w;
+        }
       }
-    }
-    // keys -1 (outside 1..nbins or x >= Nh) only lead or trail the valid lanes
-    if (__ballot(key >= 0) == 0) continue;
+      // keys -1 (outside 1..nbins or x >= Nh) only lead or trail the valid lanes
+      if (__ballot(key >= 0) == 0) continue;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int ku = __shfl_up(key, off, 64);
-      const double u0 = __shfl_up(v0, off, 64), u1 = __shfl_up(v1, off, 64),
-                   u2 = __shfl_up(v2, off, 64);
-      if (lane >= off && ku == key) { v0 += u0; v1 += u1; v2 += u2; }
-    }
-    const int kn = __shfl_down(key, 1, 64);
-    if (key >= 0 && (lane == 63 || kn != key)) {
-      acc[key] += v0;
-      acc[nbins + key] += v1;
-      acc[2 * nbins + key] += v2;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int ku = __shfl_up(key, off, 64);
+        const double u0 = __shfl_up(v0, off, 64), u1 = __shfl_up(v1, off, 64),
+                     u2 = __shfl_up(v2, off, 64);
+        if (lane >= off && ku == key) { v0 += u0; v1 += u1; v2 += u2; }
+      }
+      const int kn = __shfl_down(key, 1, 64);
+      if (key >= 0 && (lane == 63 || kn != key)) {
+        acc[key] += v0;
+        acc[nbins + key] += v1;
+        acc[2 * nbins + key] += v2;
+      }
     }
   }
   __syncthreads();
@@ -106,9 +110,7 @@ int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins,
                      int prec, double* out3) {
   PSX_REQUIRE(nbins >= 1 && nbins <= BP_MAX_BINS, "nbins must be in [1, 1024]");
   const size_t lds = (size_t)BP_WAVES * 3 * nbins * sizeof(double);
-  const int Nh = N / 2 + 1;
-  const int64_t items = (int64_t)N * N * ((Nh + 63) / 64);
-  const int blocks = (int)std::min<int64_t>(cdiv(items, BP_WAVES), BP_MAX_BLOCKS);
+  const int blocks = (int)std::min<int64_t>(cdiv((int64_t)N * N, BP_WAVES), BP_MAX_BLOCKS);
   void* part = nullptr;
   PSX_TRY(ctx_buffer(ctx, "bin_partials", (size_t)blocks * 3 * nbins * sizeof(double), &part));
   if (prec == PSX_PREC_F64) {
