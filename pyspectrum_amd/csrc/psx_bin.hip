@@ -146,8 +146,17 @@ int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k)
 }
 
 // --------------------------------------------------------- pk_pbox_rsd ----
-// Literal restatement of the mode loop of estimator.f:196-244 over the full
-// grid: x <= N/2 read directly, x > N/2 from the mirrored element.  Every
+// Restatement of the mode loop of estimator.f:196-244.  The Fortran visits
+// the full grid, x > N/2 through the mirrored element; that mode is -k' of
+// the half-field mode k' = (N - x, -y, -z), and in its float32 operations
+// every sign flip is an exact negation, so it has the same rk, k bin, |mu|,
+// mu bin, Legendre weights and |delta|^2.  The kernel therefore visits the
+// half field once and counts 0 < x < N/2 twice (x = 0 and x = N/2 map to
+// themselves): the same bins and counts, sums equal to round-off.  Except on
+// the y or z Nyquist planes: there the mirror's component is +N/2 (index N/2
+// maps to itself), not the negation, and the cos(pi/2) residue of the
+// float32 line of sight makes its |mu| differ, so that mirror is replayed
+// with the Fortran's own frequencies.  Every
 // float32 operation that decides a bin is replayed in the reference's order
 // without contraction (k bin: nint(Nbin*rk/(N/2)); mu bin:
 // int((|mu| + dmu)/dmu) in float64 of float32 values).
@@ -171,40 +180,15 @@ __global__ __launch_bounds__(256) void k_pk_rsd(const cplx<T>* __restrict__ Hh, 
     __syncthreads();
   }
   const int Nh = N / 2 + 1;
-  const int64_t n = (int64_t)N * N * N;
+  const int64_t n = (int64_t)Nh * N * N;
   const float fnb = (float)nb, fh = (float)(N / 2);
   const double md = (double)c.mubin;
   for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
        id += (int64_t)gridDim.x * blockDim.x) {
-    const int ix = (int)(id % N);
-    const int iy = (int)((id / N) % N);
-    const int iz = (int)(id / ((int64_t)N * N));
-    float rk, mu32;
-    int imk;
-    {
-#pragma clang fp contract(off)
-      const float rkx = (float)fort_freq(ix, N), rky = (float)fort_freq(iy, N), rkz = (float)fort_freq(iz, N);
-      rk = cr_sqrtf(rkx * rkx + rky * rky + rkz * rkz);
-      imk = (int)roundf(cr_divf(fnb * rk, fh));  // NINT: half away from zero
-      if (!(imk <= nb && imk != 0)) continue;
-      const float cot1 = cr_divf(rkz, rk);
-      const float sit1 = cr_sqrtf(1.0f - cot1 * cot1);
-      float cc = 0.0f;
-      if (sit1 > 0.0f) {
-        const float den = rk * sit1;
-        const float cp = cr_divf(rkx, den), sp = cr_divf(rky, den);
-        cc = c.sph * sp + c.cph * cp;
-      }
-      mu32 = c.cth * cot1 + (c.sth * sit1) * cc;
-    }
-    const double mu = (double)mu32;
-    const int imu = (int)((fabs(mu) + md) / md);
-    const double mu2 = mu * mu;
-    const double Le2 = -0.5 + 1.5 * mu2;
-    const double Le4 = 0.375 - 3.75 * mu2 + 4.375 * (mu2 * mu2);
-    cplx<T> ct;
-    if (ix <= N / 2) ct = Hh[((int64_t)iz * N + iy) * Nh + ix];
-    else ct = Hh[((int64_t)((N - iz) % N) * N + (N - iy) % N) * Nh + (N - ix)];
+    const int ix = (int)(id % Nh);
+    const int iy = (int)((id / Nh) % N);
+    const int iz = (int)(id / ((int64_t)Nh * N));
+    const cplx<T> ct = Hh[id];
     double pk;
     if constexpr (sizeof(T) == 8) {
       pk = (double)ct.re * (double)ct.re + (double)ct.im * (double)ct.im;
@@ -212,20 +196,57 @@ __global__ __launch_bounds__(256) void k_pk_rsd(const cplx<T>* __restrict__ Hh, 
       const float a = hypotf(ct.re, ct.im);
       pk = (double)(a * a);
     }
-    const double kk = (double)(c.kf * rk);
-    const int b = imk - 1;
-    atomicAdd(&acc[0 * nb + b], kk);
-    atomicAdd(&acc[1 * nb + b], pk);
-    atomicAdd(&acc[2 * nb + b], pk * 5.0 * Le2);
-    atomicAdd(&acc[3 * nb + b], pk * 9.0 * Le4);
-    atomicAdd(&acc[4 * nb + b], 1.0);
-    if (imu <= nm && imu > 0) {
-      const int bm = b * nm + (imu - 1);
-      double* a2 = acc + 5 * nb;
-      atomicAdd(&a2[0 * nb * nm + bm], kk);
-      atomicAdd(&a2[1 * nb * nm + bm], fabs(mu));
-      atomicAdd(&a2[2 * nb * nm + bm], pk);
-      atomicAdd(&a2[3 * nb * nm + bm], 1.0);
+    // one mode with frequencies (kx, ky, kz) and |delta|^2 = pk, counted wm times
+    auto mode = [&](int kx, int ky, int kz, double wm) {
+      float rk, mu32;
+      int imk;
+      {
+#pragma clang fp contract(off)
+        const float rkx = (float)kx, rky = (float)ky, rkz = (float)kz;
+        rk = cr_sqrtf(rkx * rkx + rky * rky + rkz * rkz);
+        imk = (int)roundf(cr_divf(fnb * rk, fh));  // NINT: half away from zero
+        if (!(imk <= nb && imk != 0)) return;
+        const float cot1 = cr_divf(rkz, rk);
+        const float sit1 = cr_sqrtf(1.0f - cot1 * cot1);
+        float cc = 0.0f;
+        if (sit1 > 0.0f) {
+          const float den = rk * sit1;
+          const float cp = cr_divf(rkx, den), sp = cr_divf(rky, den);
+          cc = c.sph * sp + c.cph * cp;
+        }
+        mu32 = c.cth * cot1 + (c.sth * sit1) * cc;
+      }
+      const double mu = (double)mu32;
+      const int imu = (int)((fabs(mu) + md) / md);
+      const double mu2 = mu * mu;
+      const double Le2 = -0.5 + 1.5 * mu2;
+      const double Le4 = 0.375 - 3.75 * mu2 + 4.375 * (mu2 * mu2);
+      const double kk = (double)(c.kf * rk);
+      const int b = imk - 1;
+      atomicAdd(&acc[0 * nb + b], wm * kk);
+      atomicAdd(&acc[1 * nb + b], wm * pk);
+      atomicAdd(&acc[2 * nb + b], wm * (pk * 5.0 * Le2));
+      atomicAdd(&acc[3 * nb + b], wm * (pk * 9.0 * Le4));
+      atomicAdd(&acc[4 * nb + b], wm);
+      if (imu <= nm && imu > 0) {
+        const int bm = b * nm + (imu - 1);
+        double* a2 = acc + 5 * nb;
+        atomicAdd(&a2[0 * nb * nm + bm], wm * kk);
+        atomicAdd(&a2[1 * nb * nm + bm], wm * fabs(mu));
+        atomicAdd(&a2[2 * nb * nm + bm], wm * pk);
+        atomicAdd(&a2[3 * nb * nm + bm], wm);
+      }
+    };
+    const int ky = fort_freq(iy, N), kz = fort_freq(iz, N);
+    if (ix == 0 || ix == N / 2) {
+      mode(ix, ky, kz, 1.0);  // these planes map to themselves
+    } else if (iy != N / 2 && iz != N / 2) {
+      mode(ix, ky, kz, 2.0);  // the mirror is the exact negation
+    } else {
+      // on the y/z Nyquist planes the mirror's frequency there is +N/2 (index
+      // N/2 maps to itself), not the negation: replay it separately
+      mode(ix, ky, kz, 1.0);
+      mode(-ix, fort_freq((N - iy) % N, N), fort_freq((N - iz) % N, N), 1.0);
     }
   }
   if (use_lds) {
@@ -255,7 +276,7 @@ int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int
   PSX_CHECK_LAUNCH();
   const size_t lds = (size_t)ntot * sizeof(double);
   const int use_lds = lds <= 120 * 1024;
-  const int64_t n = (int64_t)N * N * N;
+  const int64_t n = (int64_t)(N / 2 + 1) * N * N;
   const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), use_lds ? 1024 : 4096);
   const size_t shm = use_lds ? lds : 0;
   if (prec == PSX_PREC_F64) {
