@@ -320,9 +320,12 @@ def run_survey(args):
     r, nbr = catalogue(4 * nd)
     kw = dict(P0_fkp=1e4, Lbox=3600., Ngrid=360, step=args.step, Ncut=args.ncut, Nmax=args.nmax,
               cosmo=cosmo)
+    # CPU baseline leg: the oracle's host restatement of the reference's
+    # catalogue preparation (pyspectrum.py:767-806) for both catalogues
+    from oracle import spectrum as S
     t0 = time.perf_counter()
-    P._survey_catalogue(d, nbd, None, 1e4, 3600., cosmo)
-    P._survey_catalogue(r, nbr, None, 1e4, 3600., cosmo)
+    S.survey_catalogue(d, nbd, None, 1e4, 3600., cosmo)
+    S.survey_catalogue(r, nbr, None, 1e4, 3600., cosmo)
     host_ms = (time.perf_counter() - t0) * 1e3
     res, times = {}, {}
     for prec in (args.precision, "fp64"):

@@ -58,6 +58,11 @@ int psx_ctx_set_stream(psx_ctx* ctx, void* hip_stream);
 int psx_ctx_synchronize(psx_ctx* ctx);
 /* Bytes of device workspace currently held by the context. */
 int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes);
+/* Free the context's workspace after the work queued on its stream has
+ * finished (plans are kept; the next call allocates what it needs again).
+ * A request loop calls it on an idle context to keep the device memory of
+ * all its contexts within a budget.  (No reference counterpart.) */
+int psx_ctx_release_workspace(psx_ctx* ctx);
 
 /* Replaces estimator.f:284-512 assign_quad (ia=ib=ic=id=0 branch), called at
  * pyspectrum.py:951 (offset 0) and :819 (offset 0.5*Ngrid).  PCS assignment
@@ -186,6 +191,13 @@ int psx_bk_contract(psx_ctx* ctx, const void* d, int ngrid, int step, int ncut, 
  * ([nmax] float64, NaN for an empty shell, 0 below Ncut//step). */
 int psx_bk_raw(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
                double* b_raw, double* p0k, int prec);
+
+/* Upper bound of the device workspace (bytes) that psx_bk_periodic on np
+ * particles allocates in a context, for admission by memory (the request
+ * loop); the caller's own buffers are not included.  (No reference
+ * counterpart: the reference allocates host arrays per call.) */
+int psx_bk_workspace_bytes(psx_ctx* ctx, int ngrid, int step, int ncut, int nmax, int64_t np,
+                           int prec, int64_t* bytes);
 
 /* Replaces _counts_Bk123 (pyspectrum.py:962-1030, cache-miss path):
  * counts[t] = sum_x I_i I_j I_l of the unit shells in FP64 (= N^3 N_tri up to

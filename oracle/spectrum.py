@@ -418,11 +418,16 @@ def shell_fields(delta_full, step, Ncut, Nmax):
     D = np.zeros((Nmax + 1, Ngrid, Ngrid, Ngrid), dtype=np.float64)
     p0k = np.zeros(Nmax)
     for j in range(j0, Nmax + 1):
-        tmp = np.where(irk == j, delta_full, 0).astype(np.complex128)
-        D[j] = np.fft.fftn(tmp).real
+        D[j] = shell_field(delta_full, irk, j)
         with np.errstate(invalid="ignore", divide="ignore"):
             p0k[j - 1] = np.sum(D[j] * D[j]) / G / Nk[j]
     return D, p0k
+
+
+def shell_field(delta_full, irk, j):
+    """One shell of pyspectrum.py:390-400: Re FFT_forward[delta * 1(irk == j)]."""
+    tmp = np.where(irk == j, delta_full, 0).astype(np.complex128)
+    return np.fft.fftn(tmp).real
 
 
 def unit_shell_fields(Ngrid, step, Ncut, Nmax):
@@ -773,3 +778,53 @@ def quad_weights(xyz32, w, ia, ib, ic, id_):
     if ic == 0 and id_ == 0:
         return ((ww * r[ia - 1]) * r[ib - 1]) / rnorm
     return ((((ww * r[ia - 1]) * r[ib - 1]) * r[ic - 1]) * r[id_ - 1]) / (rnorm * rnorm)
+
+
+# ------------------------------------------------------- host utilities -----
+def ijl_order(i_k, j_k, l_k):
+    """util.py:8-24 ('GM' order): for each distinct l ascending, each j
+    ascending among the triangles with that l, each i ascending among those
+    with (l, j): the indices of the triangles equal to (i, j, l), in index
+    order.  Restated as a plain grouping loop (test use only)."""
+    groups = {}
+    for n, key in enumerate(zip(np.ravel(l_k), np.ravel(j_k), np.ravel(i_k))):
+        groups.setdefault(tuple(int(v) for v in key), []).append(n)
+    return np.array([groups[k] for k in sorted(groups)])
+
+
+def apply_rsd(xyz, vxyz, redshift, h, omega0_m, los, Lbox):
+    """util.py:54-75: the line-of-sight coordinate becomes
+    (x + (f v + Lbox)) % Lbox with f = (1+z)/(100 E(z)), E of flat LCDM with
+    H0 = 100 h; numpy's arithmetic in the arrays' own dtype."""
+    ez = math.sqrt(omega0_m * (1.0 + redshift) ** 3 + (1.0 - omega0_m))
+    f = (1 + redshift) / (100 * np.float64(ez))
+    out = np.array(xyz, copy=True)
+    a = {"x": 0, "y": 1, "z": 2}[los]
+    shifted = out[a] + (f * np.asarray(vxyz)[a] + Lbox)
+    out[a] = shifted % Lbox
+    return out
+
+
+def survey_catalogue(radecz, nb, w, P0_fkp, Lbox, cosmo):
+    """Host part of FFT_survey_mono (pyspectrum.py:767-806), float64, with a
+    given cosmology object (comoving_distance, h): cartesian positions
+    (util.py:27-51), the 'box not big enough' check (an AssertionError),
+    Ntot = sum w before the FKP factor, FKP weights w/(1 + nb P0) and the sums
+    I12, I13, I22, I23, I33.  Returns (float32 positions, weights, Ntot,
+    (I12, I13, I22, I23, I33)); the inputs are not modified."""
+    radecz = np.asarray(radecz, dtype=np.float64)
+    nb = np.asarray(nb, dtype=np.float64)
+    n = radecz.shape[1]
+    w = np.ones(n) if w is None else np.array(w, dtype=np.float64)
+    ra = radecz[0] * (np.pi / 180.)
+    dec = radecz[1] * (np.pi / 180.)
+    rad = np.asarray(cosmo.comoving_distance(radecz[2]) * cosmo.h, dtype=np.float64)
+    xyz = np.array([rad * np.cos(dec) * np.cos(ra), rad * np.cos(dec) * np.sin(ra),
+                    rad * np.sin(dec)])
+    if np.sum(xyz.max(axis=1) >= 0.5 * Lbox) + np.sum(np.abs(xyz.min(axis=1)) >= 0.5 * Lbox):
+        raise AssertionError("box not big enough!")
+    Ntot = np.sum(w)
+    w = w * (1. / (1. + nb * P0_fkp))
+    sums = (np.sum(w ** 2), np.sum(w ** 3), np.sum(nb * w ** 2), np.sum(nb * w ** 3),
+            np.sum(nb ** 2 * w ** 3))
+    return np.ascontiguousarray(xyz.astype(np.float32)), w, Ntot, sums

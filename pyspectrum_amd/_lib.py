@@ -30,6 +30,7 @@ SIGNATURES = {
     "psx_ctx_set_stream": [_p, _p],
     "psx_ctx_synchronize": [_p],
     "psx_ctx_workspace_bytes": [_p, _c.POINTER(_i64)],
+    "psx_ctx_release_workspace": [_p],
     "psx_assign_quad": [_p, _p, _p, _i64, _i, _f, _f, _f, _f, _d, _p, _i],
     "psx_fft3d": [_p, _p, _i, _i, _i],
     "psx_fcomb_periodic": [_p, _p, _p, _i, _d, _i],
@@ -50,6 +51,7 @@ SIGNATURES = {
     "psx_bk_contract": [_p, _p, _i, _i, _i, _i, _p, _i],
     "psx_bk_raw": [_p, _p, _i, _i, _i, _i, _p, _p, _i],
     "psx_bk_counts": [_p, _i, _i, _i, _i, _p],
+    "psx_bk_workspace_bytes": [_p, _i, _i, _i, _i, _i64, _i, _c.POINTER(_i64)],
     "psx_bk_finalize": [_p, _p, _p, _p, _p, _i, _i, _d, _d, _d, _d, _d, _p],
     "psx_bk_periodic": [_p, _p, _p, _i64, _i, _d, _d, _d, _i, _i, _i, _p, _p, _d, _d, _d, _d, _d,
                         _p, _p, _i],
@@ -124,10 +126,23 @@ class Context:
         with self._lock:
             return call(name, self.handle, *args)
 
+    def bk_workspace_bytes(self, Ngrid, step, Ncut, Nmax, npart, precision):
+        """Upper bound of the workspace a B(k) call on npart particles allocates."""
+        n = _i64()
+        with self._lock:
+            call("psx_bk_workspace_bytes", self._h, Ngrid, step, Ncut, Nmax, int(npart),
+                 PREC[precision], ctypes.byref(n))
+        return n.value
+
     def workspace_bytes(self):
         n = _i64()
         call("psx_ctx_workspace_bytes", self._h, ctypes.byref(n))
         return n.value
+
+    def release_workspace(self):
+        """Free the workspace (plans stay) once the context's stream is idle."""
+        with self._lock:
+            call("psx_ctx_release_workspace", self._h)
 
     def close(self):
         if self._h:

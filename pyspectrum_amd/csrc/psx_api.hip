@@ -203,6 +203,21 @@ int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, in
       for (int y0 = 0; y0 < N; y0 += 2 * xpairs) xt.push_back({s, z, y0, 0});
   }
   sp->t_elems = off;
+  // fused x pass: line p holds shells 2p (real part) and 2p+1 (imaginary part);
+  // kx >= min(rmax + 1, N/2 + 1) is zero in T2
+  {
+    std::vector<int> items, lkxl;
+    sp->nxlines = (sp->ns + 1) / 2;
+    for (int p = 0; p < sp->nxlines; ++p) {
+      int k = 0;
+      for (int s = 2 * p; s < std::min(2 * p + 2, sp->ns); ++s) k = std::max(k, std::min(sp->rmax[s] + 1, Nh));
+      lkxl.push_back(k);
+      for (int kx = 0; kx < k; ++kx) items.insert(items.end(), {p, kx});
+    }
+    sp->nxitems = (int)(items.size() / 2);
+    PSX_TRY(upload(ctx, items, &sp->d_xitems));
+    PSX_TRY(upload(ctx, lkxl, &sp->d_xline_kxl));
+  }
   sp->nz_tiles = (int)zt.size();
   sp->ny_tiles = (int)yt.size();
   sp->nx_tiles = (int)xt.size();
@@ -346,7 +361,9 @@ static void arrange_half_tiles(std::vector<int>* prow, int nsh, int j0) {
 
 int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
                       const ContractPlan** out) {
-  const int cprec = prec;
+  const int cprec = prec;                         // plan code (precision + layout)
+  const bool fused = (prec & CT_LAYOUT_FUSED) != 0;
+  prec &= CT_LAYOUT_FUSED - 1;
   auto key = std::make_tuple(cprec, step, ncut, nmax);
   auto it = ctx->contract_plans.find(key);
   if (it != ctx->contract_plans.end()) { *out = it->second.get(); return PSX_OK; }
@@ -404,8 +421,8 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
   cp->nblk = 0;
   for (int m : ptmask) cp->nblk += __builtin_popcount(m);
   // assign pair tiles to waves: groups of WPG waves x maxpt slots, greedy by load
-  const int maxpt = contract_maxpt(prec, cp->nlt);
-  const int WPG = contract_waves_per_group(prec, cp->nlt);
+  const int maxpt = fused ? xc_maxpt() : contract_maxpt(prec, cp->nlt);
+  const int WPG = fused ? xc_waves_per_group() : contract_waves_per_group(prec, cp->nlt);
   const int ngroups = (int)cdiv(cp->npt, WPG * maxpt);
   const int nw = ngroups * WPG;
   std::vector<int> order(cp->npt);
@@ -428,13 +445,15 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
     // blocks' worth of MFMA time) plus its output blocks
     load[best] += 3 + __builtin_popcount(ptmask[pt]);
   }
-  // output slots per group
+  // output slots per group: the blocks that hold a triangle
   std::vector<int> wave_slot(nw * maxpt * 4, -1);
   std::vector<int> gslots(ngroups, 0);
   for (int w = 0; w < nw; ++w)
-    for (int q = 0; q < maxpt; ++q)
+    for (int q = 0; q < maxpt; ++q) {
+      const int pt = wave_pt[w * maxpt + q];
       for (int t = 0; t < 4; ++t)
-        if ((wave_mask[w * maxpt + q] >> t) & 1) wave_slot[(w * maxpt + q) * 4 + t] = gslots[w / WPG]++;
+        if (pt >= 0 && ((ptmask[pt] >> t) & 1)) wave_slot[(w * maxpt + q) * 4 + t] = gslots[w / WPG]++;
+    }
   cp->nslot = *std::max_element(gslots.begin(), gslots.end());
   std::vector<int> tri_loc(cp->ntri), tri_grp(cp->ntri);
   for (int t = 0; t < cp->ntri; ++t) {
@@ -508,10 +527,16 @@ static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut
   const int j0 = ncut / step;
   const int ns = nmax - j0 + 1;
   const int64_t G = (int64_t)N * N * N;
-  void* D;
-  PSX_TRY(ctx_buffer(ctx, "shells", (size_t)ns * G * rbytes(fp), &D));
-  PSX_TRY(launch_shell_fields(ctx, half, N, step, ncut, nmax, fp, D));
-  PSX_TRY(launch_contract(ctx, D, N, step, ncut, nmax, prec, braw));
+  if (xcontract_supported(ctx, N, step, ncut, nmax, prec) == PSX_OK) {
+    // split-BF16 with the shell x pass fused into the contraction: the shell
+    // fields never go through HBM
+    PSX_TRY(launch_xcontract(ctx, half, N, step, ncut, nmax, braw));
+  } else {
+    void* D;
+    PSX_TRY(ctx_buffer(ctx, "shells", (size_t)ns * G * rbytes(fp), &D));
+    PSX_TRY(launch_shell_fields(ctx, half, N, step, ncut, nmax, fp, D));
+    PSX_TRY(launch_contract(ctx, D, N, step, ncut, nmax, prec, braw));
+  }
   if (p0k) {
     void* bins;
     PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)3 * nmax * sizeof(double), &bins));
@@ -528,7 +553,7 @@ using namespace psx;
 // ================================================================= C ABI ===
 extern "C" {
 
-int psx_abi_version(void) { return 1; }
+int psx_abi_version(void) { return 2; }
 
 const char* psx_last_error(void) { return last_error_string().c_str(); }
 
@@ -587,6 +612,20 @@ int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes) {
   int64_t s = 0;
   for (auto& kv : ctx->bufs) s += (int64_t)kv.second.bytes;
   *bytes = s;
+  return PSX_OK;
+}
+
+int psx_ctx_release_workspace(psx_ctx* ctx) {
+  PSX_REQUIRE(ctx, "null ctx");
+  PSX_CTX_LOCK(ctx);
+  PSX_HIP(hipSetDevice(ctx->device));
+  PSX_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& kv : ctx->bufs) {
+    if (kv.second.ptr) PSX_HIP(hipFree(kv.second.ptr));
+    kv.second.ptr = nullptr;
+    kv.second.bytes = 0;
+  }
+  ctx->bufs.clear();
   return PSX_OK;
 }
 
@@ -788,6 +827,43 @@ int psx_bk_raw(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, in
   PSX_REQUIRE(prec_ok(prec), "unknown precision");
   PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
   return bk_raw_impl(ctx, half, ngrid, step, ncut, nmax, b_raw, p0k, prec);
+}
+
+int psx_bk_workspace_bytes(psx_ctx* ctx, int ngrid, int step, int ncut, int nmax, int64_t np,
+                           int prec, int64_t* bytes) {
+  PSX_REQUIRE(ctx && bytes, "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(np >= 0, "negative particle count");
+  PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
+  const int fp = fft_prec(prec);
+  const int64_t N = ngrid, G = N * N * N, NN = N * N, Nh = N / 2 + 1, hN = N / 2;
+  const int j0 = ncut / step, ns = nmax - j0 + 1;
+  int64_t b = G * (int64_t)cbytes(fp) + Nh * NN * (int64_t)cbytes(fp);   // fft_grid, bk_half
+  b += 8 * (NN + 1) + 24 * np;                                             // assignment sort
+  // shell intermediates T1, T2: per shell N^2 rows of kxw values (the plan's
+  // kx extent rounded to the z/y tile width)
+  const int64_t kt = fp == PSX_PREC_F64 ? TileCfg<double>::LINES : TileCfg<float>::LINES;
+  const int64_t mgrid = 3 * hN * hN;
+  int64_t tel = 0;
+  for (int s = 0; s < ns; ++s) {
+    const int64_t mm = std::min(shell_mmax(j0 + s, step), mgrid);
+    const int64_t xlim = std::min(std::min<int64_t>(isqrt64(mm), hN) + 1, Nh);
+    tel += NN * cdiv(xlim, kt) * kt;
+  }
+  b += 2 * tel * (int64_t)cbytes(fp);
+  const bool fused = xcontract_supported(ctx, ngrid, step, ncut, nmax, prec) == PSX_OK;
+  if (!fused) b += (int64_t)ns * G * (int64_t)rbytes(fp);                  // shell fields
+  const ContractPlan* cp;
+  PSX_TRY(get_contract_plan(ctx, fused ? prec | CT_LAYOUT_FUSED : prec, step, ncut, nmax, &cp));
+  int cus = 0;
+  PSX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int64_t ngroups =
+      cp->waves / (fused ? xc_waves_per_group() : contract_waves_per_group(prec, cp->nlt));
+  b += (8 * (int64_t)cus + ngroups) * cp->nslot * 256 * 8;                // partials, reduction
+  b += (int64_t)(cp->ntri + 1) * 8 + 513 * 3 * (int64_t)nmax * 8;          // braw, p0k bins
+  *bytes = b;
+  return PSX_OK;
 }
 
 int psx_bk_counts(psx_ctx* ctx, int ngrid, int step, int ncut, int nmax, double* counts) {

@@ -156,104 +156,218 @@ int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k)
 // the y or z Nyquist planes: there the mirror's component is +N/2 (index N/2
 // maps to itself), not the negation, and the cos(pi/2) residue of the
 // float32 line of sight makes its |mu| differ, so that mirror is replayed
-// with the Fortran's own frequencies.  Every
-// float32 operation that decides a bin is replayed in the reference's order
-// without contraction (k bin: nint(Nbin*rk/(N/2)); mu bin:
+// with the Fortran's own frequencies as a second pass over those rows.
+// Every float32 operation that decides a bin is replayed in the reference's
+// order without contraction (k bin: nint(Nbin*rk/(N/2)); mu bin:
 // int((|mu| + dmu)/dmu) in float64 of float32 values).
+//
+// Deterministic: a wave takes 64 consecutive x of one (y, z) row.  The k bin
+// depends only on |k|^2 = x^2 + c, so it never decreases along the lanes;
+// a segmented scan (fixed shift order) sums each run of equal keys and the
+// run's last lane adds the run to the block's accumulators.  The (k, mu)
+// key is scanned the same way; should a key come back after another key
+// within the 64 lanes (two run ends with one key, found by a tag written and
+// read back in LDS), those run ends are added one lane at a time in lane
+// order.  The four waves of a block add their runs in turn, so the order of
+// every addition is fixed; k_bin_reduce then adds the block partials in a
+// fixed order.
 struct RsdConst {
   float kf, mubin, cth, sth, cph, sph;
   int nbin, nmu;
+  int64_t mlim;  // no |k|^2 above this has a k bin <= nbin
 };
 
 __device__ __forceinline__ int fort_freq(int n, int N) { return n <= N / 2 ? n : n - N; }
 
+// One mode (kx, ky, kz) with |delta|^2 = pk counted wm times: k bin (0-based,
+// -1 if outside 1..nbin), (k, mu) bin (-1 if outside), and the nine sums.
+struct RsdMode {
+  int kb, mb;
+  double v[9];  // k, p0, p2, p4, n | km, mk, pkm, nkm
+};
+
+__device__ __forceinline__ RsdMode rsd_mode(int kx, int ky, int kz, double pk, double wm,
+                                            const RsdConst& c, float fnb, float fh, double md) {
+  RsdMode r;
+  r.kb = -1;
+  r.mb = -1;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) r.v[e] = 0.0;
+  float rk, mu32;
+  int imk;
+  {
+#pragma clang fp contract(off)
+    const float rkx = (float)kx, rky = (float)ky, rkz = (float)kz;
+    rk = cr_sqrtf(rkx * rkx + rky * rky + rkz * rkz);
+    imk = (int)roundf(cr_divf(fnb * rk, fh));  // NINT: half away from zero
+    if (!(imk <= c.nbin && imk != 0)) return r;
+    const float cot1 = cr_divf(rkz, rk);
+    const float sit1 = cr_sqrtf(1.0f - cot1 * cot1);
+    float cc = 0.0f;
+    if (sit1 > 0.0f) {
+      const float den = rk * sit1;
+      const float cp = cr_divf(rkx, den), sp = cr_divf(rky, den);
+      cc = c.sph * sp + c.cph * cp;
+    }
+    mu32 = c.cth * cot1 + (c.sth * sit1) * cc;
+  }
+  const double mu = (double)mu32;
+  const int imu = (int)((fabs(mu) + md) / md);
+  const double mu2 = mu * mu;
+  const double Le2 = -0.5 + 1.5 * mu2;
+  const double Le4 = 0.375 - 3.75 * mu2 + 4.375 * (mu2 * mu2);
+  const double kk = (double)(c.kf * rk);
+  r.kb = imk - 1;
+  r.v[0] = wm * kk;
+  r.v[1] = wm * pk;
+  r.v[2] = wm * (pk * 5.0 * Le2);
+  r.v[3] = wm * (pk * 9.0 * Le4);
+  r.v[4] = wm;
+  if (imu <= c.nmu && imu > 0) {
+    r.mb = r.kb * c.nmu + (imu - 1);
+    r.v[5] = wm * kk;
+    r.v[6] = wm * fabs(mu);
+    r.v[7] = wm * pk;
+    r.v[8] = wm;
+  }
+  return r;
+}
+
+// Segmented inclusive scan of the NV values v[] over runs of equal key (lane
+// order, fixed shift order); returns whether this lane ends its run.
+template <int NV>
+__device__ __forceinline__ bool run_scan(int key, double* v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int ku = __shfl_up(key, off, 64);
+    double u[NV];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) u[e] = __shfl_up(v[e], off, 64);
+    if (lane >= off && ku == key) {
+#pragma unroll
+      for (int e = 0; e < NV; ++e) v[e] += u[e];
+    }
+  }
+  const int kn = __shfl_down(key, 1, 64);
+  return key >= 0 && (lane == 63 || kn != key);
+}
+
+// Adds each run end's NV sums to acc[e * stride + key]; run ends that share a
+// key (a key that came back) are added one lane at a time in lane order.
+// The lanes of a wave exchange data through LDS here (the tag written and
+// read back, the one-lane-at-a-time adds), so those accesses are volatile:
+// issued in program order, never merged or forwarded.
+template <int NV>
+__device__ __forceinline__ void run_add(bool end, int key, const double* v, double* acc,
+                                        int stride, int* tag, int lane) {
+  volatile int* vt = tag;
+  if (end) vt[key] = lane;
+  const bool clash = end && vt[key] != lane;
+  if (__ballot(clash) == 0) {
+    if (end) {
+#pragma unroll
+      for (int e = 0; e < NV; ++e) acc[e * stride + key] += v[e];
+    }
+  } else {
+    volatile double* va = acc;
+    for (int l = 0; l < 64; ++l) {
+      if (lane == l && end) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) va[e * stride + key] = va[e * stride + key] + v[e];
+      }
+    }
+  }
+}
+
+constexpr int RSD_THREADS = 256, RSD_WAVES = RSD_THREADS / 64, RSD_MAX_BLOCKS = 512;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_pk_rsd(const cplx<T>* __restrict__ Hh, int N, RsdConst c,
-                                                double* __restrict__ gout, int use_lds) {
+__global__ __launch_bounds__(RSD_THREADS) void k_pk_rsd(const cplx<T>* __restrict__ Hh, int N,
+                                                         RsdConst c, double* __restrict__ part) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // layout: [5][nbin] (k, p0, p2, p4, nk) then [4][nbin*nmu] (km, mk, pkm, nkm)
-  const int nb = c.nbin, nm = c.nmu;
-  const int ntot = 5 * nb + 4 * nb * nm;
-  double* acc = use_lds ? reinterpret_cast<double*>(smem_raw) : gout;
-  if (use_lds) {
-    for (int i = threadIdx.x; i < ntot; i += blockDim.x) acc[i] = 0.0;
-    __syncthreads();
-  }
-  const int Nh = N / 2 + 1;
-  const int64_t n = (int64_t)Nh * N * N;
-  const float fnb = (float)nb, fh = (float)(N / 2);
+  const int nb = c.nbin, nm = c.nmu, nbm = nb * nm;
+  const int ntot = 5 * nb + 4 * nbm;
+  double* acc = reinterpret_cast<double*>(smem_raw);
+  int* tagk = reinterpret_cast<int*>(acc + ntot);
+  int* tagm = tagk + nb;
+  for (int i = threadIdx.x; i < ntot; i += RSD_THREADS) acc[i] = 0.0;
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int Nh = N / 2 + 1, hN = N / 2;
+  const float fnb = (float)nb, fh = (float)hN;
   const double md = (double)c.mubin;
-  for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
-       id += (int64_t)gridDim.x * blockDim.x) {
-    const int ix = (int)(id % Nh);
-    const int iy = (int)((id / Nh) % N);
-    const int iz = (int)(id / ((int64_t)Nh * N));
-    const cplx<T> ct = Hh[id];
-    double pk;
-    if constexpr (sizeof(T) == 8) {
-      pk = (double)ct.re * (double)ct.re + (double)ct.im * (double)ct.im;
-    } else {  // the reference's float32 cabs(ct)**2
-      const float a = hypotf(ct.re, ct.im);
-      pk = (double)(a * a);
+  // work items of this wave: rows row0, row0 + rstride, ...; each row has one
+  // pass (two on the y/z Nyquist planes: the original modes, then the
+  // replayed mirrors) of 64-lane chunks x0 = 0, 64, ...
+  const int rstride = (int)gridDim.x * RSD_WAVES;
+  int row = (int)blockIdx.x * RSD_WAVES + wave, pass = 0, x0 = 0;
+  auto row_c = [&](int r) {
+    const int y = r % N, z = r / N;
+    const int ky = fort_freq(y, N), kz = fort_freq(z, N);
+    return (int64_t)ky * ky + (int64_t)kz * kz;
+  };
+  // advance to the next chunk that holds a mode below mlim
+  auto settle = [&]() {
+    while (row < N * N) {
+      const int y = row % N, z = row / N;
+      const int npass = (y == hN || z == hN) ? 2 : 1;
+      const int64_t cc = row_c(row);
+      if (pass < npass && x0 < Nh && cc + (int64_t)x0 * x0 <= c.mlim) return;
+      if (pass + 1 < npass && x0 > 0) { ++pass; x0 = 0; continue; }
+      row += rstride; pass = 0; x0 = 0;
     }
-    // one mode with frequencies (kx, ky, kz) and |delta|^2 = pk, counted wm times
-    auto mode = [&](int kx, int ky, int kz, double wm) {
-      float rk, mu32;
-      int imk;
-      {
-#pragma clang fp contract(off)
-        const float rkx = (float)kx, rky = (float)ky, rkz = (float)kz;
-        rk = cr_sqrtf(rkx * rkx + rky * rky + rkz * rkz);
-        imk = (int)roundf(cr_divf(fnb * rk, fh));  // NINT: half away from zero
-        if (!(imk <= nb && imk != 0)) return;
-        const float cot1 = cr_divf(rkz, rk);
-        const float sit1 = cr_sqrtf(1.0f - cot1 * cot1);
-        float cc = 0.0f;
-        if (sit1 > 0.0f) {
-          const float den = rk * sit1;
-          const float cp = cr_divf(rkx, den), sp = cr_divf(rky, den);
-          cc = c.sph * sp + c.cph * cp;
+  };
+  settle();
+  __syncthreads();
+  for (;;) {
+    const bool has = row < N * N;
+    if (!__syncthreads_or(has)) break;
+    RsdMode m;
+    m.kb = -1; m.mb = -1;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m.v[e] = 0.0;
+    if (has) {
+      const int y = row % N, z = row / N;
+      const int x = x0 + lane;
+      if (x < Nh) {
+        const cplx<T> ct = Hh[(int64_t)row * Nh + x];
+        double pk;
+        if constexpr (sizeof(T) == 8) {
+          pk = (double)ct.re * (double)ct.re + (double)ct.im * (double)ct.im;
+        } else {  // the reference's float32 cabs(ct)**2
+          const float a = hypotf(ct.re, ct.im);
+          pk = (double)(a * a);
         }
-        mu32 = c.cth * cot1 + (c.sth * sit1) * cc;
+        const int ky = fort_freq(y, N), kz = fort_freq(z, N);
+        const bool nyq = (y == hN || z == hN);
+        if (pass == 0) {
+          // x = 0 and x = N/2 map to themselves; elsewhere the mirror is the
+          // exact negation (weight 2) except on the Nyquist planes
+          const double wm = (x == 0 || x == hN || nyq) ? 1.0 : 2.0;
+          m = rsd_mode(x, ky, kz, pk, wm, c, fnb, fh, md);
+        } else if (x != 0 && x != hN) {
+          // the mirror on a y/z Nyquist plane, with the Fortran's frequencies
+          m = rsd_mode(-x, fort_freq((N - y) % N, N), fort_freq((N - z) % N, N), pk, 1.0, c, fnb,
+                       fh, md);
+        }
       }
-      const double mu = (double)mu32;
-      const int imu = (int)((fabs(mu) + md) / md);
-      const double mu2 = mu * mu;
-      const double Le2 = -0.5 + 1.5 * mu2;
-      const double Le4 = 0.375 - 3.75 * mu2 + 4.375 * (mu2 * mu2);
-      const double kk = (double)(c.kf * rk);
-      const int b = imk - 1;
-      atomicAdd(&acc[0 * nb + b], wm * kk);
-      atomicAdd(&acc[1 * nb + b], wm * pk);
-      atomicAdd(&acc[2 * nb + b], wm * (pk * 5.0 * Le2));
-      atomicAdd(&acc[3 * nb + b], wm * (pk * 9.0 * Le4));
-      atomicAdd(&acc[4 * nb + b], wm);
-      if (imu <= nm && imu > 0) {
-        const int bm = b * nm + (imu - 1);
-        double* a2 = acc + 5 * nb;
-        atomicAdd(&a2[0 * nb * nm + bm], wm * kk);
-        atomicAdd(&a2[1 * nb * nm + bm], wm * fabs(mu));
-        atomicAdd(&a2[2 * nb * nm + bm], wm * pk);
-        atomicAdd(&a2[3 * nb * nm + bm], wm);
+    }
+    const bool endk = run_scan<5>(m.kb, m.v, lane);
+    const bool endm = run_scan<4>(m.mb, m.v + 5, lane);
+    for (int w = 0; w < RSD_WAVES; ++w) {
+      if (w == wave && has) {
+        run_add<5>(endk, m.kb, m.v, acc, nb, tagk, lane);
+        run_add<4>(endm, m.mb, m.v + 5, acc + 5 * nb, nbm, tagm, lane);
       }
-    };
-    const int ky = fort_freq(iy, N), kz = fort_freq(iz, N);
-    if (ix == 0 || ix == N / 2) {
-      mode(ix, ky, kz, 1.0);  // these planes map to themselves
-    } else if (iy != N / 2 && iz != N / 2) {
-      mode(ix, ky, kz, 2.0);  // the mirror is the exact negation
-    } else {
-      // on the y/z Nyquist planes the mirror's frequency there is +N/2 (index
-      // N/2 maps to itself), not the negation: replay it separately
-      mode(ix, ky, kz, 1.0);
-      mode(-ix, fort_freq((N - iy) % N, N), fort_freq((N - iz) % N, N), 1.0);
+      __syncthreads();
+    }
+    if (has) {
+      x0 += 64;
+      settle();
     }
   }
-  if (use_lds) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < ntot; i += blockDim.x)
-      if (acc[i] != 0.0) atomicAdd(&gout[i], acc[i]);
-  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ntot; i += RSD_THREADS) part[(int64_t)blockIdx.x * ntot + i] = acc[i];
 }
 
 int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int nbin, int nmu,
@@ -271,23 +385,28 @@ int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int
   c.cth = (float)cos((double)th); c.sth = (float)sin((double)th);
   c.cph = (float)cos((double)ph); c.sph = (float)sin((double)ph);
   c.nbin = nbin; c.nmu = nmu;
+  // imk = nint(nbin*rk/(N/2)) <= nbin needs rk < (nbin + 1/2)(N/2)/nbin; the
+  // bound below adds a margin far above float32 rounding
+  const double rmax = (nbin + 0.5) * (N / 2) / (double)nbin * (1.0 + 1e-5) + 1e-3;
+  c.mlim = (int64_t)std::ceil(rmax * rmax) + 1;
   const int64_t ntot = 5 * (int64_t)nbin + 4 * (int64_t)nbin * nmu;
-  hipLaunchKernelGGL(k_fill_f64, dim3(64), dim3(256), 0, ctx->stream, gout, ntot);
-  PSX_CHECK_LAUNCH();
-  const size_t lds = (size_t)ntot * sizeof(double);
-  const int use_lds = lds <= 120 * 1024;
-  const int64_t n = (int64_t)(N / 2 + 1) * N * N;
-  const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), use_lds ? 1024 : 4096);
-  const size_t shm = use_lds ? lds : 0;
+  const size_t lds = (size_t)ntot * sizeof(double) + (size_t)(nbin + nbin * nmu) * sizeof(int);
+  PSX_REQUIRE(lds <= 160 * 1024, "Nbin*Nmu too large for the binning kernel (LDS)");
+  const int blocks = (int)std::min<int64_t>(cdiv((int64_t)N * N, RSD_WAVES), RSD_MAX_BLOCKS);
+  void* part = nullptr;
+  PSX_TRY(ctx_buffer(ctx, "rsd_partials", (size_t)blocks * ntot * sizeof(double), &part));
   if (prec == PSX_PREC_F64) {
-    if (shm > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_pk_rsd<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(k_pk_rsd<double>, dim3(blocks), dim3(256), shm, ctx->stream,
-                       (const cplx<double>*)half, N, c, gout, use_lds);
+    PSX_TRY(set_lds(k_pk_rsd<double>, lds));
+    hipLaunchKernelGGL(k_pk_rsd<double>, dim3(blocks), dim3(RSD_THREADS), lds, ctx->stream,
+                       (const cplx<double>*)half, N, c, (double*)part);
   } else {
-    if (shm > 65536) PSX_HIP(hipFuncSetAttribute((const void*)k_pk_rsd<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(k_pk_rsd<float>, dim3(blocks), dim3(256), shm, ctx->stream,
-                       (const cplx<float>*)half, N, c, gout, use_lds);
+    PSX_TRY(set_lds(k_pk_rsd<float>, lds));
+    hipLaunchKernelGGL(k_pk_rsd<float>, dim3(blocks), dim3(RSD_THREADS), lds, ctx->stream,
+                       (const cplx<float>*)half, N, c, (double*)part);
   }
+  PSX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_bin_reduce, dim3((unsigned)ntot), dim3(64), 0, ctx->stream,
+                     (const double*)part, blocks, (int)ntot, gout);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }

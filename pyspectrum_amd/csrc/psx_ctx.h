@@ -93,6 +93,11 @@ struct ShellPlan {
   int* d_meta = nullptr;                // per slot: j, rmax, kxw, (int)mmax -- device copy
   int64_t* d_off = nullptr;
   int64_t* d_mmax = nullptr;
+  // x pass fused with the contraction: shells 2p and 2p+1 share FFT line p;
+  // one item (p, kx) per nonzero input column kx < kxl of line p
+  int nxlines = 0, nxitems = 0;
+  int* d_xitems = nullptr;              // [nxitems][2] (p, kx)
+  int* d_xline_kxl = nullptr;           // [nxlines] kxl of line p
 };
 
 // MFMA contraction plan (pairs x shells, 16x16 blocks restricted to triangles).
@@ -134,8 +139,13 @@ int ctx_alloc_owned(psx_ctx* ctx, size_t bytes, void** out);
 int get_fft_plan(psx_ctx* ctx, int N, const FftPlan** out);
 int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, int xpairs,
                    const ShellPlan** out);
+// Plan code: the pipeline precision, plus CT_LAYOUT_FUSED for the wave layout
+// of the split-BF16 contraction fused with the shell x pass (psx_xcontract.hip).
+constexpr int CT_LAYOUT_FUSED = 16;
 int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
                       const ContractPlan** out);
+int xc_waves_per_group();
+int xc_maxpt();
 
 template <typename T> inline const cplx<T>* plan_tw(const FftPlan* p);
 template <> inline const cplx<float>* plan_tw<float>(const FftPlan* p) { return (const cplx<float>*)p->tw32; }
@@ -170,6 +180,16 @@ int launch_assign_grid(psx_ctx* ctx, const float* xyz, const float* w, int64_t n
 int fixed_point_exponent(double wsum_abs);
 int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                         int prec, void* dout);
+// z and y passes of the FP32 shell fields only (the x pass is fused into the
+// contraction): returns the plan and the intermediate T2.
+int launch_shell_zy(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                    const ShellPlan** sp_out, const void** t2_out);
+// Split-BF16 contraction with the shell x pass fused (PSX_ERR_UNSUPPORTED when
+// the shape is outside what the fused kernel covers; the caller then runs the
+// separate stages).
+int xcontract_supported(psx_ctx* ctx, int N, int step, int ncut, int nmax, int prec);
+int launch_xcontract(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                     double* braw);
 int launch_fill_unit(psx_ctx* ctx, double* half, int64_t n);
 int launch_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, const double* w,
                             int64_t n, const double* dist_nodes, const double* dist_slopes, int nseg,
@@ -182,4 +202,8 @@ int launch_five_delta2_2(psx_ctx* ctx, const void* g, void* xx, const void* xy, 
 int launch_build_quad(psx_ctx* ctx, const void* d1, void* d2, int irsd, int N, int fp);
 int launch_quad_weights(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ia, int ib,
                         int ic, int id, float* out);
+// Fixed-order float64 reduction of contraction partials [ngroups][nwg][nslot][256]
+// and the gather of each triangle's sum (reference loop order).
+int launch_contract_reduce(psx_ctx* ctx, const ContractPlan* cp, const double* part, int64_t nwg,
+                           int ngroups, double* braw);
 }  // namespace psx

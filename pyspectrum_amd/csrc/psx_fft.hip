@@ -312,7 +312,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
 
 template <typename T, int NC>
 static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, const cplx<T>* H,
-                          int step, cplx<T>* t1, cplx<T>* t2, T* dout) {
+                          int step, cplx<T>* t1, cplx<T>* t2, T* dout, bool do_x = true) {
   const int N = p->shape.N;
   ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
   const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
@@ -330,7 +330,7 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
                        (const cplx<T>*)t1, t2, sp->d_ytiles, sd, p->shape, tw);
     PSX_CHECK_LAUNCH();
   }
-  if (sp->nx_tiles > 0) {
+  if (do_x && sp->nx_tiles > 0) {
     PSX_TRY(set_lds(k_shell_x<T, NC>, ldsx));
     hipLaunchKernelGGL((k_shell_x<T, NC>), dim3(sp->nx_tiles), dim3(FFT_THREADS), ldsx, ctx->stream,
                        (const cplx<T>*)t2, dout, sp->d_xtiles, sd, p->shape, tw,
@@ -363,6 +363,34 @@ static int shell_fields_t(psx_ctx* ctx, const cplx<T>* H, int N, int step, int n
   switch (N) {
     PSX_FFT_SIZES(PSX_CASE_N)
     default: return shell_fields_n<T, 0>(ctx, p, sp, H, step, (cplx<T>*)t1, (cplx<T>*)t2, (T*)dout);
+  }
+}
+#undef PSX_CASE_N
+
+#define PSX_CASE_N(n) \
+  case n: return shell_fields_n<float, n>(ctx, p, sp, H, step, t1, t2, nullptr, false);
+
+int launch_shell_zy(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                    const ShellPlan** sp_out, const void** t2_out) {
+  const FftPlan* p;
+  PSX_TRY(get_fft_plan(ctx, N, &p));
+  int nc = 0;
+#define PSX_IF_N(n) if (N == n) nc = n;
+  PSX_FFT_SIZES(PSX_IF_N)
+#undef PSX_IF_N
+  const ShellPlan* sp;
+  PSX_TRY(get_shell_plan(ctx, N, step, ncut, nmax, TileCfg<float>::LINES, shell_x_pairs<float>(nc), &sp));
+  void *v1 = nullptr, *v2 = nullptr;
+  PSX_TRY(ctx_buffer(ctx, "shell_t1_f32", (size_t)sp->t_elems * sizeof(cplx<float>), &v1));
+  PSX_TRY(ctx_buffer(ctx, "shell_t2_f32", (size_t)sp->t_elems * sizeof(cplx<float>), &v2));
+  *sp_out = sp;
+  *t2_out = v2;
+  const cplx<float>* H = (const cplx<float>*)half;
+  cplx<float>* t1 = (cplx<float>*)v1;
+  cplx<float>* t2 = (cplx<float>*)v2;
+  switch (N) {
+    PSX_FFT_SIZES(PSX_CASE_N)
+    default: return shell_fields_n<float, 0>(ctx, p, sp, H, step, t1, t2, nullptr, false);
   }
 }
 #undef PSX_CASE_N

@@ -439,39 +439,6 @@ def _Bk_periodic(delta, Nmax=40, Ncut=3, step=3, fft='pyfftw', nthreads=1, silen
 
 
 # ------------------------------------------------------- survey geometry ----
-def _survey_catalogue(radecz, nb, w, P0_fkp, Lbox, cosmo):
-    """Host part of FFT_survey_mono (pyspectrum.py:767-806), float64: cartesian
-    positions, the 'box not big enough' check, Ntot = sum w (before FKP), FKP
-    weights w/(1 + nb P0) and the normalisation sums I12, I13, I22, I23, I33.
-    The caller's arrays are not modified (the reference scales w in place)."""
-    if isinstance(radecz, torch.Tensor):
-        radecz = radecz.detach().cpu().numpy()
-    radecz = np.asarray(radecz)
-    N = radecz.shape[1]
-    nb = np.asarray(nb.detach().cpu().numpy() if isinstance(nb, torch.Tensor) else nb,
-                    dtype=np.float64)
-    if w is None:
-        w = np.ones(N)
-    else:
-        w = np.array(w.detach().cpu().numpy() if isinstance(w, torch.Tensor) else w,
-                     dtype=np.float64)
-    xyz = util.radecz_to_cartesian(radecz, cosmo=cosmo)
-    xyz_max = np.max(xyz, axis=1)
-    xyz_min = np.min(xyz, axis=1)
-    assert np.sum(xyz_max >= 0.5 * Lbox) + np.sum(np.abs(xyz_min) >= 0.5 * Lbox) == 0, \
-        'box not big enough!'
-    xyzs = np.ascontiguousarray(xyz.astype(np.float32))
-    Ntot = np.sum(w)
-    w = w * (1. / (1. + nb * P0_fkp))
-    I12 = np.sum(w ** 2)
-    I13 = np.sum(w ** 3)
-    I22 = np.sum(nb * w ** 2)
-    I23 = np.sum(nb * w ** 3)
-    I33 = np.sum(nb ** 2 * w ** 3)
-    assert np.all([(I12 >= 0), (I13 >= 0), (I22 >= 0), (I23 >= 0), (I33 >= 0)])
-    return xyzs, w, Ntot, (I12, I13, I22, I23, I33)
-
-
 def _survey_half_on_device(radecz, nb, w, P0_fkp, Lbox, Ngrid, cosmo, prec, device):
     """FFT_survey_mono on the device: psx_survey_catalogue (positions, FKP
     weights, normalisation sums), the box check, then psx_fft_survey_mono."""

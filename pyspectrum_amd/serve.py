@@ -4,16 +4,18 @@ Each request is one periodic box (positions already in HBM) asking for
 P0/P2/P4 with the line of sight along z (Pk_periodic_rsd, pyspectrum.py:460)
 and B(k) (Bk_periodic, pyspectrum.py:285).  The loop
 
-* admits a request when a worker slot is free and its HBM footprint
-  (``footprint_bytes``) fits the budget next to the requests in flight;
+* admits a request when a worker slot is free and the device memory of all
+  slots stays within the HBM budget: a busy slot counts the larger of its
+  request's footprint (``footprint_bytes``) and the workspace its context
+  still holds from earlier requests, an idle slot its held workspace; when a
+  request does not fit, the workspace of idle slots is released first;
 * runs admitted requests concurrently, one library context and one
   default-priority stream per slot, so their kernels overlap on the device;
 * computes the half field once per request and reuses it for P_l and B;
 * records arrival, start and completion times (latency = completion - arrival).
 
-Every stage is deterministic except the float64 atomics of the P_l binning
-(pk_pbox_rsd), so a request's B(k) outputs are bit-identical whether it ran
-alone or in the mix, and its P_l agrees to float64 round-off.
+Every stage sums in a fixed order, so a request's outputs are bit-identical
+whether it ran alone or in the mix.
 """
 import ctypes
 import math
@@ -28,16 +30,17 @@ from . import _lib as L
 from . import pyspectrum as P
 
 
-def footprint_bytes(Ngrid, npart, Nmax=40, step=3, Ncut=3, precision="fp32"):
-    """Upper estimate of the device memory a request holds (workspace + I/O)."""
-    G = float(Ngrid) ** 3
-    r = 8 if precision == "fp64" else 4
-    ns = Nmax - Ncut // step + 1
-    kx = min(Ngrid // 2 + 1, step * (Nmax + 1) + 16)
-    grids = 2 * r * G + 2 * r * G / 2                      # packed FFT grid, half field
-    shells = ns * G * r + 2 * 2 * r * Ngrid * Ngrid * kx * ns / 2   # D + T1/T2 (average kx)
-    parts = 24.0 * npart                                   # tile keys and ranks, sorted records
-    return int(grids + shells + parts + (64 << 20))
+def footprint_bytes(Ngrid, npart, Nmax=40, step=3, Ncut=3, precision="fp32", Nmubin=10, device=0):
+    """Upper estimate of the device memory one request holds: the library's
+    workspace bound for its B(k) call (psx_bk_workspace_bytes), the P_l
+    binning partials, and the request's own buffers (half field, outputs)."""
+    ws = L.context(device).bk_workspace_bytes(Ngrid, step, Ncut, Nmax, npart, precision)
+    nb = Ngrid // 2
+    c = 16 if precision == "fp64" else 8
+    half = (Ngrid // 2 + 1) * Ngrid * Ngrid * c
+    rsd = 513 * (5 * nb + 4 * nb * Nmubin) * 8
+    ntri = len(L.triangles(step, Ncut, Nmax))
+    return int(ws + half + rsd + 16 * ntri * 8 + (1 << 20))
 
 
 @dataclass
@@ -68,6 +71,10 @@ class Slot:
         self.graphs = {}
         with torch.cuda.device(device):
             self.stream = torch.cuda.Stream(device=device)
+
+    def held_bytes(self):
+        """Device workspace this slot keeps between requests (its context and graphs)."""
+        return self.ctx.workspace_bytes() + sum(g.workspace_bytes() for g in self.graphs.values())
 
     def add_graph(self, key):
         from .graphs import BkGraph
@@ -125,13 +132,14 @@ class Server:
         self.device = device
         free, total = torch.cuda.mem_get_info(device)
         self.budget = int(hbm_budget if hbm_budget is not None else 0.8 * free)
-        self._fixed_budget = hbm_budget is not None
         self.use_graphs = use_graphs
         self.graph_build_s = 0.0
         self.slots = [Slot(device) for _ in range(nslots)]
         self._lock = threading.Condition()
-        self._inflight_bytes = 0
+        self._held = [0] * nslots      # workspace each slot's context holds (bytes)
+        self._need = [0] * nslots      # footprint of the request a busy slot runs
         self._free_slots = list(range(nslots))
+        self.peak_bytes = 0            # largest committed total at any admission
 
     def close(self):
         """Release the slots' graphs and library contexts."""
@@ -142,19 +150,44 @@ class Server:
             slot.ctx.close()
         torch.cuda.empty_cache()
 
+    def _committed(self, s=None, need=0):
+        """Device bytes of all slots, with slot ``s`` running a request of
+        footprint ``need``: busy slots max(held, footprint), idle slots held."""
+        tot = 0
+        for t in range(len(self.slots)):
+            n = need if t == s else self._need[t]
+            tot += max(self._held[t], n)
+        return tot
+
     def _admit(self, need):
         with self._lock:
-            while not self._free_slots or (self._inflight_bytes + need > self.budget
-                                           and self._inflight_bytes > 0):
+            while True:
+                if self._free_slots:
+                    # prefer the free slot whose held workspace covers most of the request
+                    s = max(self._free_slots, key=lambda t: (min(self._held[t], need), -t))
+                    if self._committed(s, need) > self.budget:
+                        # release idle workspace (not the chosen slot's: it is reused)
+                        for t in self._free_slots:
+                            if t != s and self._held[t] > 0:
+                                self.slots[t].ctx.release_workspace()
+                                self._held[t] = self.slots[t].held_bytes()
+                    if self._committed(s, need) > self.budget and self._held[s] > 0:
+                        self.slots[s].ctx.release_workspace()
+                        self._held[s] = self.slots[s].held_bytes()
+                    busy = any(n > 0 for n in self._need)
+                    if self._committed(s, need) <= self.budget or not busy:
+                        self._free_slots.remove(s)
+                        self._need[s] = need
+                        self.peak_bytes = max(self.peak_bytes, self._committed())
+                        return s
                 self._lock.wait()
-            s = self._free_slots.pop()
-            self._inflight_bytes += need
-            return s
 
     def _release(self, s, need):
+        held = self.slots[s].held_bytes()
         with self._lock:
+            self._held[s] = held
+            self._need[s] = 0
             self._free_slots.append(s)
-            self._inflight_bytes -= need
             self._lock.notify_all()
 
     def serve(self, requests):
@@ -181,9 +214,9 @@ class Server:
                         pass  # unsupported shape: its requests fail and report it
             torch.cuda.synchronize(self.device)
             self.graph_build_s += time.perf_counter() - tg
-            if not self._fixed_budget:
-                self.budget = int(0.8 * torch.cuda.mem_get_info(self.device)[0])
         torch.cuda.synchronize(self.device)
+        with self._lock:
+            self._held = [slot.held_bytes() for slot in self.slots]
         threads = []
         t0 = time.perf_counter()
 
@@ -203,8 +236,11 @@ class Server:
             delay = req.arrival - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
-            need = footprint_bytes(req.Ngrid, req.xyz.shape[1], req.Nmax, req.step, req.Ncut,
-                                   req.precision)
+            try:
+                need = footprint_bytes(req.Ngrid, req.xyz.shape[1], req.Nmax, req.step, req.Ncut,
+                                       req.precision, req.Nmubin, self.device)
+            except (L.PsxError, ValueError):
+                need = 0  # an unsupported request fails on its slot and records the error
             s = self._admit(need)
             th = threading.Thread(target=work, args=(req, s, need), daemon=True)
             th.start()

@@ -1,16 +1,18 @@
 """Build-time autotuner for the compile-time tile shapes and FFT radix order.
 
 The kernels are specialised at compile time -- waves per workgroup and pair
-tiles per wave of the split-BF16 contraction, points per chunk, row pairs per
-tile of the shell x pass, lines per FFT tile, the radix order -- so a tuning
-choice is a set of preprocessor definitions.  Two steps:
+tiles per wave of the split-BF16 contractions (fused and staged), rows per
+float32 accumulation chain, row pairs per tile of the shell x pass, lines per
+FFT tile, the radix order -- so a tuning choice is a set of preprocessor
+definitions.  Two steps:
 
     python -m pyspectrum_amd.tune build            # CPU: one library per candidate
     python -m pyspectrum_amd.tune time [--ngrid 360 --nmax 40 ...]   # GPU
 
 ``build`` compiles every candidate of ``CANDIDATES`` into
 ``pyspectrum_amd/_tune/<name>/libpsx.so``.  ``time`` times the B(k) stages of
-each candidate (FFT_periodic, shell fields, contraction; device events,
+each candidate (FFT_periodic, then shell fields + contraction + shell powers
+as psx_bk_raw runs them; device events,
 median of repeats) on a seeded lognormal box, each in its own process (a
 process loads one library), over several interleaved rounds, and writes the
 fastest candidate's definitions -- the default unless another is faster by
@@ -32,17 +34,16 @@ TUNE_DIR = os.path.join(HERE, "_tune")
 # name -> definitions; the empty set is the built-in default
 CANDIDATES = {
     "default": {},
-    "ct_w8x3": {"PSX_CT_WAVES_BF": 8, "PSX_CT_MAXPT_BF": 3},
-    "ct_kc128": {"PSX_CT_KC_BF": 128},
-    "ct_w8x3_kc128": {"PSX_CT_WAVES_BF": 8, "PSX_CT_MAXPT_BF": 3, "PSX_CT_KC_BF": 128},
-    "ct_kc4_256": {"PSX_CT_KC_BF4": 256},
-    "ct_kc4_64": {"PSX_CT_KC_BF4": 64},
-    "ct_f32_w8": {"PSX_CT_WAVES_F32": 8},
-    "ct_f32_kc256": {"PSX_CT_KC_F32": 256},
-    "shellx_p8": {"PSX_SHELL_X_PAIRS": 8},
-    "shellx_p4": {"PSX_SHELL_X_PAIRS": 4},
+    # split-BF16 contraction fused with the shell x pass (psx_xcontract.hip)
+    "xc_flush2": {"PSX_XC_FLUSH": 2},
+    "xc_flush4": {"PSX_XC_FLUSH": 4},
+    "xc_w12x2": {"PSX_XC_WAVES": 12, "PSX_XC_MAXPT": 2},
+    # FFT tiles and radix order (front end, shell z/y passes, fused x pass)
     "fft_lines8": {"PSX_TILE_LINES_F32": 8},
     "fft_order_9_5_8": {"PSX_FFT_ORDER": 2},
+    # the staged contraction (grids or shell counts outside the fused kernel)
+    "ct_w8x3": {"PSX_CT_WAVES_BF": 8, "PSX_CT_MAXPT_BF": 3},
+    "shellx_p8": {"PSX_SHELL_X_PAIRS": 8},
 }
 
 
@@ -73,19 +74,17 @@ def _time_one(lib, ngrid, npart, lbox, step, ncut, nmax, prec, reps, seed):
     ctx = L.Context(0)
     fp = L.PREC[prec]
     cd = torch.complex128 if prec == "fp64" else torch.complex64
-    rd = torch.float64 if prec == "fp64" else torch.float32
-    ns = nmax - ncut // step + 1
     half = torch.empty((ngrid, ngrid, ngrid // 2 + 1), dtype=cd, device=dev)
-    D = torch.empty((ns, ngrid, ngrid, ngrid), dtype=rd, device=dev)
     braw = torch.empty(max(len(L.triangles(step, ncut, nmax)), 1), dtype=torch.float64, device=dev)
+    p0k = torch.empty(nmax, dtype=torch.float64, device=dev)
     n = xyz.shape[1]
+    # the two halves of a box: the front end, and shell fields + contraction +
+    # shell powers (fused x pass and contraction where the shape allows)
     stages = {
         "fft_periodic": lambda h: L.call("psx_fft_periodic", h, L.ptr(xyz), None, n, ngrid,
                                          float(lbox), float(n), float(n), L.ptr(half), fp),
-        "shell_fields": lambda h: L.call("psx_bk_shell_fields", h, L.ptr(half), ngrid, step, ncut,
-                                         nmax, L.ptr(D), fp),
-        "contraction": lambda h: L.call("psx_bk_contract", h, L.ptr(D), ngrid, step, ncut, nmax,
-                                        L.ptr(braw), fp),
+        "bk_raw": lambda h: L.call("psx_bk_raw", h, L.ptr(half), ngrid, step, ncut, nmax,
+                                   L.ptr(braw), L.ptr(p0k), fp),
     }
     out = {}
     for name, fn in stages.items():

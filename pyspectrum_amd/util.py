@@ -96,50 +96,62 @@ def radecz_to_cartesian(radecz, cosmo=None):
     (util.py:27-51).  Unlike the reference, the input array is not converted
     to radians in place."""
     radecz = np.asarray(radecz)
-    assert radecz.shape[0] == 3, "radecz has to be have shape [3,N]"
-    if cosmo is None:
-        cosmo = default_cosmology()
-    ra = np.array(radecz[0], dtype=np.float64) * (np.pi / 180.)
-    dec = np.array(radecz[1], dtype=np.float64) * (np.pi / 180.)
-    z = np.array(radecz[2], dtype=np.float64)
-    rad = np.asarray(cosmo.comoving_distance(z) * cosmo.h, dtype=np.float64)
-    return np.array([rad * np.cos(dec) * np.cos(ra),
-                     rad * np.cos(dec) * np.sin(ra),
-                     rad * np.sin(dec)])
+    if radecz.ndim < 1 or radecz.shape[0] != 3:
+        raise AssertionError("radecz must be a (3, N) array of RA, Dec, z")
+    cosmo = default_cosmology() if cosmo is None else cosmo
+    ra, dec = np.radians(np.asarray(radecz[0], dtype=np.float64)), np.radians(
+        np.asarray(radecz[1], dtype=np.float64))
+    r = np.asarray(cosmo.comoving_distance(np.asarray(radecz[2], dtype=np.float64)) * cosmo.h,
+                   dtype=np.float64)
+    rxy = r * np.cos(dec)
+    return np.stack([rxy * np.cos(ra), rxy * np.sin(ra), r * np.sin(dec)])
 
 
 def ijl_order(i_k, j_k, l_k, typ='GM'):
-    """Indices that reorder triangles by (l, j, i) ascending (util.py:8-24)."""
+    """Triangle indices sorted by (l, j, i) ascending (util.py:8-24).
+
+    One row per distinct triangle, holding the positions where it occurs in
+    increasing order (a single column when every triangle occurs once).
+    """
     if typ != 'GM':
-        raise NotImplementedError
-    i_k, j_k, l_k = np.asarray(i_k), np.asarray(j_k), np.asarray(l_k)
-    i_bq = np.arange(len(i_k))
-    out = []
-    for l in np.sort(np.unique(l_k)):
-        for j in np.sort(np.unique(j_k[l_k == l])):
-            for i in np.sort(np.unique(i_k[(l_k == l) & (j_k == j)])):
-                out.append(i_bq[(i_k == i) & (j_k == j) & (l_k == l)])
-    return np.array(out)
+        raise NotImplementedError(typ)
+    keys = np.stack([np.ravel(i_k), np.ravel(j_k), np.ravel(l_k)])
+    if keys.shape[1] == 0:
+        return np.zeros(0, dtype=np.int64)
+    perm = np.lexsort(keys)  # last key primary: l, then j, then i; stable
+    ks = keys[:, perm]
+    first = np.r_[True, np.any(ks[:, 1:] != ks[:, :-1], axis=0)]
+    sizes = np.diff(np.r_[np.flatnonzero(first), perm.size])
+    if np.any(sizes != sizes[0]):
+        raise ValueError("triangles repeat an unequal number of times")
+    return perm.reshape(-1, int(sizes[0]))
+
+
+_LOS_AXIS = {'x': 0, 'y': 1, 'z': 2}
+
+
+def rsd_shift(redshift, h=0.7, omega0_m=0.3):
+    """Position shift per unit velocity, (1+z)/(100 E(z)) [Mpc/h per km/s]."""
+    ez = FlatLambdaCDM(H0=100. * h, Om0=omega0_m).efunc(redshift)
+    return (1 + redshift) / (100 * ez)
 
 
 def applyRSD(xyz, vxyz, redshift, h=0.7, omega0_m=0.3, LOS=None, Lbox=None):
-    """Redshift-space positions in a periodic box (util.py:54-75):
-    x_los += (1+z)/(100 E(z)) v_los, wrapped into [0, Lbox)."""
-    xyz = np.asarray(xyz)
-    vxyz = np.asarray(vxyz)
-    assert xyz.shape[0] == 3
-    assert vxyz.shape[0] == 3
+    """Redshift-space positions in a periodic box (util.py:54-75): along the
+    line of sight x <- (x + (s v + Lbox)) mod Lbox with s = rsd_shift(z), the
+    reference's order of floating-point operations.  Shapes are asserted as
+    in the reference; LOS and Lbox are required keywords."""
+    pos, vel = np.asarray(xyz), np.asarray(vxyz)
+    if pos.shape[0] != 3 or vel.shape[0] != 3:
+        raise AssertionError("positions and velocities must be (3, N) arrays")
     if LOS is None:
-        raise ValueError("specify line of sight")
+        raise ValueError("LOS (line of sight: 'x', 'y' or 'z') is required")
     if Lbox is None:
-        raise ValueError("specify box size")
-    i_los = {'x': 0, 'y': 1, 'z': 2}[LOS]
-    cosmo = FlatLambdaCDM(H0=100. * h, Om0=omega0_m)
-    rsd_factor = (1 + redshift) / (100 * cosmo.efunc(redshift))
-    xyz_rsd = xyz.copy()
-    xyz_rsd[i_los] += rsd_factor * vxyz[i_los] + Lbox
-    xyz_rsd[i_los] = (xyz_rsd[i_los] % Lbox)
-    return xyz_rsd
+        raise ValueError("Lbox (box size) is required")
+    ax = _LOS_AXIS[LOS]
+    out = np.array(pos, copy=True)
+    out[ax] = np.mod(out[ax] + (rsd_shift(redshift, h, omega0_m) * vel[ax] + Lbox), Lbox)
+    return out
 
 
 # ------------------------------------------------------------ text outputs --

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert L.lib().psx_abi_version() == 1
+    assert L.lib().psx_abi_version() == 2
 
 
 @pytest.mark.parametrize("step,ncut,nmax", [(3, 3, 10), (3, 3, 40), (2, 2, 60), (1, 1, 5), (3, 3, 30)])

@@ -1,0 +1,130 @@
+"""GPU tests of the request loop (pyspectrum_amd/serve.py) at the sizes of the
+measured workloads:
+
+* the batch of 64 uneven lognormal boxes (seeds 100-163, N log-uniform in
+  [2e5, 4e6], Ngrid in {128, 192, 256, 360}, Lbox in {1000, 2000, 2500},
+  step 3, Ncut 3, Nmax 40) submitted at once to 4 slots: every output of
+  every box bit-identical to the same box run alone, and split-BF16 against
+  FP64 for one box of each grid size (the tolerances at the test);
+* the serving stream with its largest requests (Ngrid 512, 4e6 redshift-space
+  points) mixed with smaller ones: alone = mix bit for bit;
+* the HBM budget: the device memory held by all slots (request footprints
+  and the workspace contexts keep between requests) stays within it.
+"""
+import numpy as np
+import pytest
+import torch
+
+from pyspectrum_amd import mocks
+from pyspectrum_amd import serve
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("k", "p0k", "p2k", "p4k", "counts", "b123", "q123", "p0k1", "p0k2", "p0k3", "b_counts",
+        "b123_sn")
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _alone(r, precision=None, server=None):
+    ref = serve.Request(r.rid, r.xyz, r.Lbox, r.Ngrid, step=r.step, Ncut=r.Ncut, Nmax=r.Nmax,
+                        Nmubin=r.Nmubin, precision=precision or r.precision)
+    out, _ = server.serve([ref])
+    assert out[0].error is None, out[0].error
+    return out[0].result
+
+
+def _same(a, b):
+    for k in KEYS:
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+def test_batch_of_64_boxes_alone_equals_mix():
+    _say("batch 64 boxes, 4 slots")
+    reqs = serve.make_stream(64, rate=0.0, seed=100, grids=(128, 192, 256, 360))
+    server = serve.Server(nslots=4)
+    mix, wall = server.serve(reqs)
+    assert all(r.error is None for r in mix), [r.error for r in mix if r.error]
+    _say("  %d boxes in %.3f s" % (len(mix), wall))
+    server.close()
+    alone = serve.Server(nslots=1)
+    for r in mix:
+        _same(_alone(r, server=alone), r.result)
+    # split-BF16 against FP64, one box per grid size.  Triangles whose shells
+    # all lie inside the Nyquist sphere (|k| <= Ngrid/2): B within the
+    # required 1e-3 of max(|B|, B_sn) and B_raw within the 2e-5 regression
+    # bound, per triangle.  Shells past it (Ngrid 128 and 192 at Nmax 40) hold
+    # a few modes near the cube's corners; their normalised sums are ratios of
+    # small counts and cancel to the FP32 FFTs' round-off in every FP32
+    # pipeline, so there only a coarse bound (1e-2) guards against gross errors.
+    # Shells beyond the grid are empty (NaN powers, the reference's behaviour).
+    from pyspectrum_amd import _lib as L
+    for N in (128, 192, 256, 360):
+        r = [q for q in mix if q.Ngrid == N][0]
+        a, b = r.result, _alone(r, "fp64", server=alone)
+        big = L.triangles(r.step, r.Ncut, r.Nmax)[:, 0]
+        ok = (b["b_counts"] > 0) & np.isfinite(b["b123"]) & np.isfinite(a["b123"])
+        inside = ok & ((big + 0.5) * r.step <= N / 2)
+        den = np.maximum(np.abs(b["b123"]), np.abs(b["b123_sn"]))
+        eb = np.abs(a["b123"] - b["b123"]) / den
+        raw_a, raw_b = a["b123"] + a["b123_sn"], b["b123"] + b["b123_sn"]
+        er = np.abs(raw_a - raw_b) / np.abs(raw_b)
+        _say("  Ngrid %d: inside B %.2e B_raw %.2e (%d triangles); all B %.2e"
+             % (N, eb[inside].max(), er[inside].max(), inside.sum(), eb[ok].max()))
+        assert inside.sum() > 100
+        assert eb[inside].max() < 1e-3 and er[inside].max() < 2e-5, N
+        assert eb[ok].max() < 1e-2, N
+    alone.close()
+
+
+def _rsd_box(n, Lbox, seed):
+    xyz, _ = mocks.lognormal_box(n, Lbox, seed=seed, mesh=256)
+    v = mocks.gaussian_velocities(n, 300.0, seed=seed)
+    return mocks.apply_rsd(xyz, v, 0.5, Lbox, los=2)
+
+
+def test_serving_largest_requests_alone_equals_mix():
+    _say("serving stream with Ngrid 512, 4e6 points")
+    reqs = serve.make_stream(4, rate=20.0, seed=5, grids=(128, 256), nrange=(2e5, 1e6), mesh=256)
+    t = reqs[-1].arrival
+    for k, seed in enumerate((5001, 5002)):
+        reqs.append(serve.Request(len(reqs), _rsd_box(4_000_000, 3000.0, seed), 3000.0, 512,
+                                  arrival=t + 0.01 * (k + 1)))
+    server = serve.Server(nslots=2)
+    mix, _ = server.serve(reqs)
+    assert all(r.error is None for r in mix), [r.error for r in mix if r.error]
+    server.close()
+    alone = serve.Server(nslots=1)
+    for r in mix:
+        _same(_alone(r, server=alone), r.result)
+    alone.close()
+
+
+def test_slots_stay_within_hbm_budget():
+    """Mixed grids on three slots under a budget that holds about two of the
+    largest requests: the committed total (footprints of running requests and
+    workspace held by every slot) never exceeds it, the workspace each slot
+    holds afterwards fits it too, and each request's own workspace is within
+    its footprint estimate."""
+    _say("serving HBM budget")
+    reqs = serve.make_stream(8, rate=0.0, seed=11, grids=(128, 256, 360), nrange=(2e5, 1e6),
+                             mesh=128)
+    need = max(serve.footprint_bytes(r.Ngrid, r.xyz.shape[1], r.Nmax, r.step, r.Ncut, r.precision)
+               for r in reqs)
+    budget = int(2.2 * need)
+    server = serve.Server(nslots=3, hbm_budget=budget)
+    out, _ = server.serve(reqs)
+    assert all(r.error is None for r in out)
+    assert server.peak_bytes <= budget
+    assert sum(s.held_bytes() for s in server.slots) <= budget
+    server.close()
+    one = serve.Server(nslots=1)
+    for r in reqs:
+        one.slots[0].ctx.release_workspace()
+        _alone(r, server=one)
+        fp = serve.footprint_bytes(r.Ngrid, r.xyz.shape[1], r.Nmax, r.step, r.Ncut, r.precision)
+        assert one.slots[0].held_bytes() <= fp, (r.Ngrid, one.slots[0].held_bytes(), fp)
+    one.close()
+    torch.cuda.empty_cache()

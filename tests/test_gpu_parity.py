@@ -21,6 +21,10 @@ from pyspectrum_amd import pyspectrum as P
 pytestmark = pytest.mark.gpu
 
 TOL = {"fp64": 1e-10, "fp32": 1e-5, "bf16x3": 1e-3}
+# Regression bounds near the measured split-BF16 error (about 3e-6 of FP64 on
+# B_raw): the 1e-3 requirement alone would let a 100x loss of accuracy pass,
+# such as dropping the per-chunk float64 carry of the MFMA sums (about 2e-4).
+BF16_REGRESSION = 2e-5
 DEV = torch.device("cuda", 0)
 
 
@@ -212,7 +216,11 @@ def test_bk_raw_matches_oracle(prec, N, step, ncut, nmax):
     got = braw.cpu().numpy()
     # error relative to the typical triangle magnitude (the sums change sign)
     scale = np.max(np.abs(ref))
-    assert np.max(np.abs(got - ref)) / scale < TOL[prec]
+    err = np.max(np.abs(got - ref)) / scale
+    _say("  err %.2e" % err)
+    assert err < TOL[prec]
+    if prec == "bf16x3":
+        assert err < BF16_REGRESSION
     # p0k by Parseval from the half field (float32 in the fp32/bf16x3 pipelines)
     j0 = max(ncut // step, 1)
     assert _rel(p0.cpu().numpy()[j0 - 1:], p0k[j0 - 1:]) < (1e-12 if prec == "fp64" else 1e-6)
@@ -257,7 +265,7 @@ def _fixture(golden_dir, nmax):
     return {(a, b, c): (n, v) for a, b, c, n, v in zip(g["i"], g["j"], g["l"], g["ntri"], g["counts"])}
 
 
-@pytest.mark.parametrize("N,nmax", [(96, 10), (360, 10), (360, 40)])
+@pytest.mark.parametrize("N,nmax", [(96, 10), (360, 10), (360, 40), (360, 50)])
 def test_ntri_bit_exact_against_reference_cache(golden_dir, N, nmax):
     _say("ntri", N, nmax)
     ijl = L.triangles(3, 3, nmax)
@@ -266,7 +274,9 @@ def test_ntri_bit_exact_against_reference_cache(golden_dir, N, nmax):
     c = out.cpu().numpy()
     G = float(N) ** 3
     ntri = np.rint(c / G).astype(np.int64)
-    assert np.max(np.abs(c / G - ntri)) < 1e-3
+    # FP64 sums of N^3 terms: off an integer by far less than the 0.5 that
+    # rint resolves (absolute 1e-3, or 1e-12 of the count for counts ~1e10)
+    assert np.all(np.abs(c / G - ntri) < np.maximum(1e-3, 1e-12 * ntri))
     fx = _fixture(golden_dir, nmax)
     want = np.array([fx[tuple(t)][0] for t in ijl])
     assert np.array_equal(ntri, want)
@@ -399,11 +409,15 @@ def test_precision_stress_config(nmax, ntri):
     keep = ref["counts"] > 0
     braw64 = (ref["b123"] + ref["b123_sn"])[keep]
     den = np.maximum(np.abs(ref["b123"]), np.abs(ref["b123_sn"]))[keep]
-    for p, tol, tol_raw in (("fp32", 1e-5, 1e-4), ("bf16x3", 1e-3, 1e-3)):
+    # split-BF16 B_raw: 1e-3 required; measured about 2e-5 of FP64 here (the
+    # FP32 shell FFTs set that floor), bounded at 1e-4 against regressions
+    for p, tol, tol_raw in (("fp32", 1e-5, 1e-4), ("bf16x3", 1e-3, 1e-4)):
         got = out[p]
         assert np.array_equal(got["counts"], ref["counts"]), p
         braw = (got["b123"] + got["b123_sn"])[keep]
-        assert np.max(np.abs(braw - braw64) / np.abs(braw64)) < tol_raw, p
+        e_raw = np.max(np.abs(braw - braw64) / np.abs(braw64))
+        _say("  %s B_raw err %.2e" % (p, e_raw))
+        assert e_raw < tol_raw, p
         assert np.max(np.abs(got["b123"][keep] - ref["b123"][keep]) / den) < tol, p
         assert _rel(got["p0k1"], ref["p0k1"]) < tol, p
 
@@ -424,6 +438,25 @@ def test_bk_periodic_weighted_matches_oracle(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pk_periodic_rsd_large_grid_and_determinism(prec):
+    # several 64-lane chunks per row and Nyquist planes of a 144^3 grid
+    # against the oracle; two calls agree bit for bit (fixed-order sums)
+    _say("Pk_periodic_rsd 144", prec)
+    N, Lbox = 144, 1000
+    xyz, _ = _particles(60000, Lbox, 24)
+    ref = S.Pk_periodic_rsd(xyz, None, Lbox, N, 2, 10)
+    got = [P.Pk_periodic_rsd(xyz, Lbox=Lbox, Ngrid=N, rsd=2, Nmubin=10, precision=prec)
+           for _ in range(2)]
+    assert np.array_equal(got[0]["counts"], ref["counts"])
+    assert np.array_equal(got[0]["counts_kmu"], ref["counts_kmu"])
+    tol = 1e-9 if prec == "fp64" else 1e-5
+    for k in ("k", "p0k", "k_kmu", "mu_kmu", "p_kmu"):
+        assert _rel(got[0][k], ref[k]) < tol, k
+    for k in ("k", "p0k", "p2k", "p4k", "counts", "k_kmu", "mu_kmu", "p_kmu", "counts_kmu"):
+        assert np.array_equal(got[0][k], got[1][k], equal_nan=True), k
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_pk_periodic_rsd_weighted_matches_oracle(prec):
     _say("Pk_periodic_rsd weighted", prec)
     N, Lbox = 40, 700
@@ -434,3 +467,27 @@ def test_pk_periodic_rsd_weighted_matches_oracle(prec):
     tol = 1e-9 if prec == "fp64" else 1e-5
     for k in ("k", "p0k", "p_kmu"):
         assert _rel(got[k], ref[k]) < tol, k
+
+
+# --------------------------------------------------- units of B(k), P(k) -----
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_bk_units_scale_and_shot_noise(prec):
+    """The device finalisation (k_bk_finalize) against dimensions and
+    constants: positions and Lbox x 2 leave every float32 grid coordinate
+    unchanged, so B scales by 2^6, P by 2^3 and Q is unchanged (1e-12); for
+    uniform randoms the shot-noise-subtracted P and B scatter around zero
+    (|mean P/P_sn| < 0.05, |mean B/B_sn| < 0.5), which fixes the powers of
+    2 pi in (2 pi)^3/kf^3 and (2 pi)^6/kf^6."""
+    _say("B(k) units", prec)
+    N, Lbox = 48, 1000.0
+    rng = np.random.default_rng(42)
+    xyz = rng.uniform(0, Lbox, size=(3, 400000))
+    a = P.Bk_periodic(xyz, Lbox=Lbox, Ngrid=N, step=3, Ncut=3, Nmax=8, precision=prec)
+    b = P.Bk_periodic(2.0 * xyz, Lbox=2.0 * Lbox, Ngrid=N, step=3, Ncut=3, Nmax=8, precision=prec)
+    ok = a["counts"] > 0
+    assert np.allclose(b["b123"][ok], 64.0 * a["b123"][ok], rtol=1e-12, atol=0)
+    assert np.allclose(b["p0k1"][ok], 8.0 * a["p0k1"][ok], rtol=1e-12, atol=0)
+    assert np.allclose(b["q123"][ok], a["q123"][ok], rtol=1e-12, atol=0)
+    nbar = a["meta"]["nbar"]
+    assert abs(np.mean(a["p0k1"][ok]) * nbar) < 0.05
+    assert abs(np.mean(a["b123"][ok] / a["b123_sn"][ok])) < 0.5

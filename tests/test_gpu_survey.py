@@ -85,8 +85,8 @@ def test_b0_survey_matches_oracle(prec):
     # fields, and p0k, b123 subtract shot-noise terms of their own size; the
     # float32 pipelines (like the reference's complex64 one) carry the float32
     # rounding of those large terms, so errors are measured against them
-    _, _, Nd, _ = P._survey_catalogue(d, nb, w, 1e4, Lbox, util.default_cosmology())
-    _, _, Nr, Ir = P._survey_catalogue(r, nbr, None, 1e4, Lbox, util.default_cosmology())
+    _, _, Nd, _ = S.survey_catalogue(d, nb, w, 1e4, Lbox, util.default_cosmology())
+    _, _, Nr, Ir = S.survey_catalogue(r, nbr, None, 1e4, Lbox, util.default_cosmology())
     alpha = Nd / Nr
     sn_p = (1. + alpha) * Ir[0] / Ir[2]
     floors = {"p0k1": sn_p, "b123": np.max(np.abs(ref["b123"])),
@@ -119,7 +119,7 @@ def test_b0_survey_identical_catalogues_closed_form():
     d, nb, w = _catalogue(3000, 25)
     got = P.B0_survey(d.copy(), nb, w=w, radecz_r=d.copy(), nbar_r=nb, w_r=w, P0_fkp=1e4, Lbox=Lbox,
                       Ngrid=N, Nmax=Nmax, precision="fp32")
-    _, _, _, I = P._survey_catalogue(d, nb, w, 1e4, Lbox, util.default_cosmology())
+    _, _, _, I = S.survey_catalogue(d, nb, w, 1e4, Lbox, util.default_cosmology())
     I12, I13, I22, I23, I33 = I
     valid = got["counts"] > 0
     assert np.allclose(got["p0k1"][valid], -2 * I12 / I22, rtol=1e-13)
@@ -154,7 +154,7 @@ def test_device_catalogue_matches_host():
     _say("device survey catalogue")
     d, nb, w = _catalogue(50000, 28, zlo=0.0, zhi=0.6)
     c = util.default_cosmology()
-    xyz_h, w_h, Ntot_h, I_h = P._survey_catalogue(d, nb, w, 1e4, 4000.0, c)
+    xyz_h, w_h, Ntot_h, I_h = S.survey_catalogue(d, nb, w, 1e4, 4000.0, c)
     dev = torch.device("cuda", 0)
     rd = torch.from_numpy(np.ascontiguousarray(d)).to(dev)
     nbd = torch.from_numpy(nb).to(dev)

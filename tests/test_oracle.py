@@ -306,3 +306,39 @@ def test_pk_periodic_parseval_against_shells():
     nbar = 800 / Lbox ** 3
     want = p0k * (2 * np.pi) ** 3 / kf ** 3 - 1 / nbar
     assert np.allclose(pk["p0k"], want, rtol=1e-10)
+
+
+# --------------------------------------------------- units of B(k), P(k) -----
+def test_bk_units_scale_with_box_size():
+    """bk_finalize's unit conversion (pyspectrum.py:346-353) by dimensions:
+    positions and Lbox scaled by s = 2 leave every float32 grid coordinate
+    unchanged, so B must scale as s^6, P as s^3 and Q not at all.  An
+    exponent slip in (2 pi)^6/kf^6 or (2 pi)^3/kf^3 breaks this."""
+    rng = np.random.default_rng(41)
+    N, Lbox = 24, 600.0
+    xyz = rng.uniform(0, Lbox, size=(3, 30000))
+    counts = S.counts_Bk123(N, 6, 2, 2)
+    counts[np.rint(counts / N ** 3) == 0] = 0.0
+    a = S.Bk_periodic(xyz, None, Lbox, N, 2, 2, 6, counts=counts)
+    b = S.Bk_periodic(2.0 * xyz, None, 2.0 * Lbox, N, 2, 2, 6, counts=counts)
+    ok = a["counts"] > 0
+    assert np.allclose(b["b123"][ok], 64.0 * a["b123"][ok], rtol=1e-12, atol=0)
+    assert np.allclose(b["b123_sn"][ok], 64.0 * a["b123_sn"][ok], rtol=1e-12, atol=0)
+    assert np.allclose(b["p0k1"][ok], 8.0 * a["p0k1"][ok], rtol=1e-12, atol=0)
+    assert np.allclose(b["q123"][ok], a["q123"][ok], rtol=1e-12, atol=0)
+
+
+def test_bk_units_uniform_randoms_are_shot_noise():
+    """The constants of the unit conversion: for Poisson-sampled uniform
+    points the shot-noise-subtracted P(k) and B(k) scatter around zero, i.e.
+    P_raw (2 pi)^3/kf^3 = 1/nbar and B_raw (2 pi)^6/kf^6 = B_sn on average.
+    A wrong power of 2 pi would shift P/P_sn by >= 5 and B/B_sn by >= 5."""
+    rng = np.random.default_rng(42)
+    N, Lbox = 48, 1000.0
+    xyz = rng.uniform(0, Lbox, size=(3, 400000))
+    counts = S.counts_Bk123(N, 8, 3, 3)
+    counts[np.rint(counts / N ** 3) == 0] = 0.0
+    b = S.Bk_periodic(xyz, None, Lbox, N, 3, 3, 8, counts=counts)
+    nbar = b["meta"]["nbar"]
+    assert abs(np.mean(b["p0k1"]) * nbar) < 0.05
+    assert abs(np.mean(b["b123"] / b["b123_sn"])) < 0.5

@@ -113,7 +113,7 @@ def test_survey_catalogue_sums_match_oracle():
     radecz = np.array([rng.uniform(0, 60, n), rng.uniform(-20, 20, n), rng.uniform(0.05, 0.12, n)])
     nb = rng.uniform(1e-4, 5e-4, n)
     w = rng.uniform(0.8, 1.2, n)
-    xyzs, wf, Ntot, I = P._survey_catalogue(radecz, nb, w, 1e4, 1000.0, util.default_cosmology())
+    xyzs, wf, Ntot, I = S.survey_catalogue(radecz, nb, w, 1e4, 1000.0, util.default_cosmology())
     ref_w = w * (1. / (1. + nb * 1e4))
     assert Ntot == np.sum(w)
     assert np.array_equal(wf, ref_w)
@@ -122,7 +122,7 @@ def test_survey_catalogue_sums_match_oracle():
     assert tuple(I) == exp
     assert xyzs.dtype == np.float32
     with pytest.raises(AssertionError):
-        P._survey_catalogue(radecz, nb, w, 1e4, 200.0, util.default_cosmology())
+        S.survey_catalogue(radecz, nb, w, 1e4, 200.0, util.default_cosmology())
 
 
 def test_ijl_order_sorts_by_l_then_j_then_i():
@@ -159,3 +159,35 @@ def test_driver_text_tables_round_trip(tmp_path):
     bk["i_k1"] = bk["i_k1"][:-1]
     with pytest.raises(ValueError):
         util.write_bk_text(str(tmp_path / "bad.dat"), bk, 2500.0)
+
+
+def test_ijl_order_matches_reference_grouping():
+    # bit-level: the lexsort form against the oracle's restatement of the
+    # reference's grouping loop, on a shuffled triangle list and on a list in
+    # which every triangle occurs twice
+    I, J, Lx, _ = S.triangles(12, 3, 3)
+    rng = np.random.default_rng(5)
+    p = rng.permutation(len(I))
+    for a, b, c in ((I[p], J[p], Lx[p]), (np.r_[I, I][::-1], np.r_[J, J][::-1], np.r_[Lx, Lx][::-1])):
+        got = util.ijl_order(a, b, c)
+        want = S.ijl_order(a, b, c)
+        assert got.shape == want.shape and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("los", ["x", "y", "z"])
+def test_apply_rsd_matches_reference_arithmetic(dtype, los):
+    # util.applyRSD bit for bit against the oracle's restatement of
+    # util.py:54-75 (float64 shift factor, x + (f v + Lbox), then % Lbox)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0, 1000, (3, 20000)).astype(dtype)
+    x[:, :3] = [[0.0], [999.999], [500.0]]
+    v = rng.normal(0, 300, (3, 20000)).astype(dtype)
+    v[:, 0] = -3e5  # a shift longer than the box: negative before the wrap
+    got = util.applyRSD(x, v, 0.5, h=0.7, omega0_m=0.3, LOS=los, Lbox=1000.0)
+    want = S.apply_rsd(x, v, 0.5, 0.7, 0.3, los, 1000.0)
+    assert got.dtype == x.dtype and np.array_equal(got, want)
+    other = [a for a in range(3) if a != "xyz".index(los)]
+    assert np.array_equal(got[other], x[other])
+    with pytest.raises(ValueError):
+        util.applyRSD(x, v, 0.5, LOS=los)
