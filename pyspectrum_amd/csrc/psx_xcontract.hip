@@ -388,7 +388,9 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
 }
 
 // Compile-time sizes of the fused path and the line-count classes.
+#ifndef PSX_XC_SIZES
 #define PSX_XC_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(360) X(384)
+#endif
 
 static int xc_nlines_ct(int ns) {
   const int nl = (ns + 1) / 2;

@@ -18,7 +18,14 @@ _counts_Bk123          pyspectrum.py:962-1030                         psx_bk_cou
 
 Extra keyword ``precision`` selects the pipeline: ``'fp32'`` (default: FP32
 FFTs, FP32 MFMA contraction, FP64 cross-tile sums), ``'bf16x3'`` (split-BF16
-three-term MFMA contraction) or ``'fp64'``.  ``fft`` and ``nthreads`` are
+three-term MFMA contraction) or ``'fp64'``.  The split-BF16 bound (B within
+1e-3 of FP64, relative to max(|B|, B_sn)) holds for triangles whose shells
+lie inside the Nyquist sphere ((j + 1/2) step <= Ngrid/2, e.g. every shell
+at Ngrid 360, step 3, Nmax 40).  Shells past it hold only modes near the
+cube's corners, whose triangle sums cancel by orders of magnitude; there the
+16 significant bits of each split operand give errors up to about 1e-2
+(six times the FP32 pipeline's, measured at Ngrid 128), so such grids want
+``'fp32'`` or ``'fp64'``.  ``fft`` and ``nthreads`` are
 accepted and ignored (the reference's ``'fortran'``/``'numpy'`` FFT branches
 are broken, pyspectrum.py:1076-1079).
 
