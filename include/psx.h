@@ -165,6 +165,16 @@ int psx_hermitian_half(psx_ctx* ctx, const void* full, int in_prec, int64_t sx, 
 int psx_bin_power(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins, double kf,
                   double* out3, int prec);
 
+/* psx_bin_power with no device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same arguments, kernels and results (bit
+ * for bit) as psx_bin_power; its block partials live in a fixed 6 MiB device
+ * array that the library holds on each device until the process ends, so
+ * nbins is limited to 1..512 (PSX_ERR_ARG beyond).  Calls that overlap on one
+ * device share that array and race: serialise them. */
+int psx_bin_power_noalloc(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins,
+                          double kf, double* out3, int prec);
+
 /* Replaces estimator.f:155-264 pk_pbox_rsd (pyspectrum.py:634).  out: float64
  * sums [5][nbin] (k, p0, p2, p4, nk) then [4][nbin][nmu] (km, mk, pkm, nkm),
  * before the division by counts and kf^3 (done by the caller). */

@@ -44,6 +44,7 @@ SIGNATURES = {
     "psx_build_quad": [_p, _p, _p, _i, _i, _i],
     "psx_hermitian_half": [_p, _p, _i, _i64, _i64, _i64, _p, _i, _i],
     "psx_bin_power": [_p, _p, _i, _i, _i, _d, _p, _i],
+    "psx_bin_power_noalloc": [_p, _p, _i, _i, _i, _d, _p, _i],
     "psx_pk_pbox_rsd": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
     "psx_bk_num_triangles": [_i, _i, _i, _c.POINTER(_i64)],
     "psx_bk_triangles": [_i, _i, _i, _p],

@@ -17,6 +17,8 @@ int launch_herm_half(psx_ctx* ctx, const void* F, int in_prec, int64_t sx, int64
 int launch_reflect(psx_ctx* ctx, const void* Hh, void* F, int N, int prec);
 int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
                      int prec, double* out3);
+int launch_bin_power_noalloc(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
+                             int prec, double* out3);
 int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int nbin, int nmu,
                   int prec, double* gout);
 int launch_contract(psx_ctx* ctx, const void* D, int N, int step, int ncut, int nmax, int prec,
@@ -773,6 +775,16 @@ int psx_bin_power(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins
   PSX_REQUIRE(step >= 1, "step >= 1 required");
   PSX_TRY(check_ngrid(ngrid));
   return launch_bin_power(ctx, half, ngrid, step, nbins, kf, fft_prec(prec), out3);
+}
+
+int psx_bin_power_noalloc(psx_ctx* ctx, const void* half, int ngrid, int step, int nbins,
+                          double kf, double* out3, int prec) {
+  PSX_REQUIRE(ctx && half && out3, "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_REQUIRE(step >= 1, "step >= 1 required");
+  PSX_TRY(check_ngrid(ngrid));
+  return launch_bin_power_noalloc(ctx, half, ngrid, step, nbins, kf, fft_prec(prec), out3);
 }
 
 int psx_pk_pbox_rsd(psx_ctx* ctx, const void* half, int irsd, int lbox, int nbin, int nmu,

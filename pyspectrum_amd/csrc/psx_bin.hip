@@ -1,6 +1,8 @@
 // k-space binning of |delta|^2: P(k) shells (Pk_periodic), the shell powers
 // of the bispectrum (Parseval form of pyspectrum.py:404) and the P_l / P(k,mu)
 // estimator of pk_pbox_rsd (estimator.f:155-264).
+#include <mutex>
+
 #include "psx_ctx.h"
 
 namespace psx {
@@ -106,13 +108,18 @@ __global__ void k_fill_f64(double* p, int64_t n) {
     p[i] = 0.0;
 }
 
-int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
-                     int prec, double* out3) {
-  PSX_REQUIRE(nbins >= 1 && nbins <= BP_MAX_BINS, "nbins must be in [1, 1024]");
+// Block partials of k_bin_power: one [3][nbins] record per block.
+static int bin_power_blocks(int N) {
+  return (int)std::min<int64_t>(cdiv((int64_t)N * N, BP_WAVES), BP_MAX_BLOCKS);
+}
+
+// k_bin_power + k_bin_reduce with the block partials in `part` (at least
+// bin_power_blocks(N) * 3 * nbins doubles; every entry is written before it
+// is read, so nothing needs clearing).
+static int bin_power_into(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
+                          int prec, double* out3, void* part) {
   const size_t lds = (size_t)BP_WAVES * 3 * nbins * sizeof(double);
-  const int blocks = (int)std::min<int64_t>(cdiv((int64_t)N * N, BP_WAVES), BP_MAX_BLOCKS);
-  void* part = nullptr;
-  PSX_TRY(ctx_buffer(ctx, "bin_partials", (size_t)blocks * 3 * nbins * sizeof(double), &part));
+  const int blocks = bin_power_blocks(N);
   if (prec == PSX_PREC_F64) {
     PSX_TRY(set_lds(k_bin_power<double>, lds));
     hipLaunchKernelGGL(k_bin_power<double>, dim3(blocks), dim3(BP_THREADS), lds, ctx->stream,
@@ -127,6 +134,63 @@ int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins,
                      blocks, 3 * nbins, out3);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
+}
+
+int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
+                     int prec, double* out3) {
+  PSX_REQUIRE(nbins >= 1 && nbins <= BP_MAX_BINS, "nbins must be in [1, 1024]");
+  void* part = nullptr;
+  PSX_TRY(ctx_buffer(ctx, "bin_partials", (size_t)bin_power_blocks(N) * 3 * nbins * sizeof(double),
+                     &part));
+  return bin_power_into(ctx, half, N, step, nbins, kf, prec, out3, part);
+}
+
+// ------------------------------------------- bin power without allocation ---
+// The block partials of psx_bin_power_noalloc live in this device array
+// (NOALLOC_DOUBLES doubles = 6 MiB: 512 blocks x 3 sums x 512 bins), which
+// each device holds until the process ends.  Host code never passes the
+// array's name to a kernel: its device address is looked up once per device
+// with hipGetSymbolAddress and kept in noalloc_ptr[device].  Every call on a
+// device shares that device's array, so calls that overlap on one device
+// race; each call writes every partial it reads, so nothing needs clearing.
+constexpr int NOALLOC_MAX_BINS = 512;
+constexpr size_t NOALLOC_DOUBLES = (size_t)BP_MAX_BLOCKS * 3 * NOALLOC_MAX_BINS;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* noalloc_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex noalloc_mu;
+
+// The device address of `scratch` on the context's device (looked up on the
+// first call with that device current).
+static int noalloc_scratch(psx_ctx* ctx, void** out) {
+  const int dev = ctx->device;
+  PSX_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    int cur = 0;
+    PSX_HIP(hipGetDevice(&cur));
+    if (cur != dev) PSX_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return set_error(PSX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return PSX_OK;
+}
+
+int launch_bin_power_noalloc(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
+                             int prec, double* out3) {
+  PSX_REQUIRE(nbins >= 1 && nbins <= NOALLOC_MAX_BINS,
+              "psx_bin_power_noalloc: nbins must be in [1, 512] (fixed scratch of 6 MiB)");
+  PSX_REQUIRE((size_t)bin_power_blocks(N) * 3 * nbins <= NOALLOC_DOUBLES,
+              "psx_bin_power_noalloc: partials exceed the fixed scratch");
+  void* part = nullptr;
+  PSX_TRY(noalloc_scratch(ctx, &part));
+  return bin_power_into(ctx, half, N, step, nbins, kf, prec, out3, part);
 }
 
 // p0k[j-1] = sum |delta|^2 / Nk[j] for j = max(j0,1)..nmax (NaN if Nk = 0),
