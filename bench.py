@@ -9,9 +9,11 @@ resident in HBM before timing; N_tri comes from the FP64 count cache (as in
 the reference), computed once before the timed region.
 
 Prints one JSON line: boxes/s of the mixed-precision (split-BF16) path, its
-p50/p99 latency, the FP64 path's boxes/s and the ratio, per-stage times, the
-roofline of the dominant kernel and a CPU baseline (the oracle, timed on a
-bounded sample of the same workload on this host).
+p50/p99 latency, the FP64 path's boxes/s and the ratio, the device time per
+box of each main kernel (HIP events the library records around its launches
+over the timed region), the roofline of the kernel with the most time and a
+CPU baseline (the reference-structured oracle path, timed on a bounded
+sample of the same box on this host, 1 thread and all cores).
 """
 import argparse
 import json
@@ -44,12 +46,14 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-triangles", type=int, default=12)
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--config", default="box", choices=["box", "serve", "batch", "pmc", "survey"],
+    ap.add_argument("--config", default="box", choices=["box", "serve", "batch", "pmc", "survey", "cpu"],
                     help="box: the single clustered box (default); serve: Poisson request "
                          "stream at --rate/s; batch: 64 uneven boxes submitted at once; "
                          "pmc: one box, a 1 GiB device copy (counter calibration), one box "
                          "-- the program to run under rocprofv3 --pmc; survey: B0_survey on a "
-                         "synthetic footprint (--npart galaxies, 4x randoms)")
+                         "synthetic footprint (--npart galaxies, 4x randoms); cpu: full host "
+                         "timings of the reference-structured CPU path (plain config at 1 "
+                         "thread and all cores, the clustered box at all cores)")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--nreq", type=int, default=40)
     ap.add_argument("--slots", type=int, default=4)
@@ -88,13 +92,23 @@ class Box:
                (2 * np.pi) ** 6, self.kf ** 6, L.ptr(self.out7), L.ptr(self.p0k), L.PREC[prec])
 
 
+# Kernels the library brackets with HIP events while kernel timing is on.
+KERNELS = ("k_assign_lines", "k_fft_lines", "k_shell_z", "k_shell_y", "k_shell_x", "k_contract",
+           "k_xcontract")
+
+
 def time_steps(box, prec, steps, warmup):
+    """Wall time and per-step latency of `steps` boxes, and the device time
+    of each main kernel over the same timed region: the library records HIP
+    events around each launch on the stream the kernels run on (the
+    context's stream, bound to torch's current stream by each call)."""
     torch = box.torch
     for _ in range(warmup):
         box.run(prec)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(steps)]
+    box.ctx.kernel_timing(True)
     t0 = time.perf_counter()
     for a, b in ev:
         a.record()
@@ -102,44 +116,15 @@ def time_steps(box, prec, steps, warmup):
         b.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kern = {}
+    for k in KERNELS:
+        n, ms = box.ctx.kernel_time(k)
+        if n:
+            kern[k] = {"launches_per_box": n / steps, "ms_per_launch": ms / n,
+                       "ms_per_box": ms / steps}
+    box.ctx.kernel_timing(False)
     lat = np.array([a.elapsed_time(b) for a, b in ev])
-    return wall, lat
-
-
-def stage_times(box, prec, reps=3):
-    """Per-stage device times (ms, median of reps) through the staged C ABI."""
-    torch, L = box.torch, box.L
-    N, dev = box.N, box.xyz.device
-    fp = L.PREC[prec]
-    cd = torch.complex128 if prec == "fp64" else torch.complex64
-    rd = torch.float64 if prec == "fp64" else torch.float32
-    j0 = box.Ncut // box.step
-    ns = box.Nmax - j0 + 1
-    half = torch.empty((N, N, N // 2 + 1), dtype=cd, device=dev)
-    D = torch.empty((ns, N, N, N), dtype=rd, device=dev)
-    braw = torch.empty(box.ntri, dtype=torch.float64, device=dev)
-    h = box.ctx.handle
-    stages = {
-        "fft_periodic": lambda: L.call("psx_fft_periodic", h, L.ptr(box.xyz), None, box.np_, N,
-                                       float(box.Lbox), float(box.np_), float(box.np_), L.ptr(half), fp),
-        "shell_fields": lambda: L.call("psx_bk_shell_fields", h, L.ptr(half), N, box.step, box.Ncut,
-                                       box.Nmax, L.ptr(D), fp),
-        "contraction": lambda: L.call("psx_bk_contract", h, L.ptr(D), N, box.step, box.Ncut,
-                                      box.Nmax, L.ptr(braw), fp),
-    }
-    out = {}
-    for name, fn in stages.items():
-        fn()
-        ts = []
-        for _ in range(reps):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(); fn(); b.record()
-            torch.cuda.synchronize()
-            ts.append(a.elapsed_time(b))
-        out[name] = float(np.median(ts))
-    del D
-    torch.cuda.empty_cache()
-    return out
+    return wall, lat, kern
 
 
 def accuracy(out, p0k, out64, p0k64, cnt):
@@ -158,87 +143,131 @@ def accuracy(out, p0k, out64, p0k64, cnt):
             "triangles": int(m.sum())}
 
 
-def contraction_traffic(N, Nmax, prec):
-    """HBM bytes per contraction launch measured with rocprofv3 --pmc
-    (FETCH_SIZE x 2 + WRITE_SIZE, the factor 2 from a 1 GiB copy kernel), kept
-    as a committed summary next to the command that produced it; None when no
-    measurement exists for this workload."""
+def kernel_traffic(kernel, N, Nmax, prec):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy),
+    kept as a committed summary next to the command that produced it; None
+    when no measurement exists for this kernel and workload."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             tab = json.load(f)
     except (OSError, ValueError):
         return None
-    v = tab.get("k_contract/%s/N%d/Nmax%d" % (prec, N, Nmax))
+    v = tab.get("%s/%s/N%d/Nmax%d" % (kernel, prec, N, Nmax))
     return None if v is None else float(v)
 
 
-def cpu_baseline(xyz_np, Lbox, Ngrid, step, Ncut, Nmax, n_tri_sample):
-    """The oracle (FP64 numpy restatement of the reference path, 1 thread) timed
-    on a bounded sample of the same box: the full front end (assignment, FFT,
-    fcomb, reflect), one shell field and n_tri_sample triangle sums; the box
-    time is extrapolated to all shells and triangles."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    from oracle import spectrum as S
-    t0 = time.perf_counter()
-    half = S.FFT_periodic(xyz_np, None, Lbox, Ngrid)
-    full = S.reflect_delta(half, Ngrid)
-    irk = S.shell_index(Ngrid, step)
-    t_front = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    d = [np.fft.fftn(np.where(irk == j, full, 0)).real.ravel() for j in (Nmax,)]
-    t_shell = time.perf_counter() - t0
-    d3 = [d[0], d[0] * 0.5, d[0] * 0.25]
-    t0 = time.perf_counter()
-    for _ in range(n_tri_sample):
-        np.einsum("i,i,i", d3[0], d3[1], d3[2])
-    t_tri = (time.perf_counter() - t0) / n_tri_sample
-    ns = Nmax - Ncut // step + 1
-    ntri = len(S.triangles(Nmax, Ncut, step)[0])
-    box_s = t_front + ns * t_shell + ntri * t_tri
-    sample_s = t_front + t_shell + n_tri_sample * t_tri
-    return {"value": 1.0 / box_s, "unit": "boxes/s", "cores": 1, "kind": "port",
-            "sample": ("oracle (FP64 numpy restatement) on the same 1e6-point box: full front end "
-                       "%.1fs, 1 of %d shell FFTs %.1fs, %d of %d triangle einsums %.3fs each "
-                       "(%.0fs sampled); box time extrapolated to %.0fs"
-                       % (t_front, ns, t_shell, n_tri_sample, ntri, t_tri, sample_s, box_s))}
+def roofline(box, prec, kern):
+    """Roofline of the kernel with the most device time per box.  The
+    triangle sums are G * (Npairs + 2 Ntri) flops per launch (one product per
+    shell pair and point, one multiply-add per triangle and point); the fused
+    kernel also runs the x pass of the shell fields, 5 N log2(N) flops per
+    complex line of N points, ceil(Nshell / 2) lines per (y, z) row."""
+    name = max(kern, key=lambda k: kern[k]["ms_per_box"])
+    N = box.N
+    G = float(N) ** 3
+    npairs = len({(int(i), int(j)) for i, j, _ in box.ijl})
+    flops = G * (npairs + 2 * box.ntri)
+    if name == "k_xcontract":
+        ns = box.Nmax - box.Ncut // box.step + 1
+        flops += 5.0 * G * np.log2(N) * ((ns + 1) // 2)
+    elif name != "k_contract":
+        return {"bound": None, "kernel": name, "achieved": None, "peak": None, "unit": None,
+                "frac": None, "traffic": kernel_traffic(name, N, box.Nmax, prec)}
+    t = kern[name]["ms_per_launch"] / 1e3
+    achieved = flops / t / 1e12
+    peak = PEAK["bf16_tflops"] if prec == "bf16x3" else (
+        PEAK["fp32_tflops"] if prec == "fp32" else PEAK["fp64_tflops"])
+    return {"bound": "mfma", "kernel": name, "achieved": round(achieved, 2), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "flops_per_launch": flops, "ms_per_launch": round(kern[name]["ms_per_launch"], 4),
+            "traffic": kernel_traffic(name, N, box.Nmax, prec)}
 
 
-def cpu_baseline_all_cores(xyz_np, Lbox, Ngrid, step, Ncut, Nmax, n_tri_sample):
-    """The same sampled oracle box with the host's cores (at most 16): the
-    FFTs by scipy.fft (pocketfft) with one worker per core, the triangle sums
-    spread over a thread pool (numpy releases the GIL in them); assignment and
-    fcomb stay vectorised numpy."""
+def host_cores():
+    """Cores this process may use on the host, at most 16 (one GPU's share)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_box(xyz_np, Lbox, Ngrid, step, Ncut, Nmax, cores, ntri_sample=None, say=None):
+    """The reference-structured CPU path (the oracle's restatement) on one box
+    with `cores` threads: float32 PCS + interlaced assignment, one complex64
+    packed c2c backward FFT (scipy.fft, `cores` workers), fcomb, the
+    Hermitian reflection, one complex64 forward FFT per shell stored as
+    float64, and one float64 triple sum per triangle (a thread pool over
+    triangles; numpy releases the GIL in them).  With ntri_sample, only the
+    first shell FFT and that many triangle sums run and the shell and
+    triangle totals are extrapolated.  Returns the stage times (s)."""
     import scipy.fft
     from concurrent.futures import ThreadPoolExecutor
     from oracle import spectrum as S
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    t = {}
     t0 = time.perf_counter()
     n = xyz_np.shape[1]
     A, B = S.assign_quad(S.clip_positions(xyz_np, Lbox), np.ones(n), Ngrid,
                          np.float32(float(Ngrid) / Lbox))
-    F = scipy.fft.ifftn(A + 1j * B, workers=cores) * A.size
-    F = S.fcomb_periodic(F, float(n), Ngrid)
+    F = scipy.fft.ifftn((A + 1j * B).astype(np.complex64), workers=cores) * np.float32(A.size)
+    del A, B
+    F = S.fcomb_periodic(F.astype(np.complex128), float(n), Ngrid)
     full = S.reflect_delta(F[: Ngrid // 2 + 1], Ngrid)
+    del F
     irk = S.shell_index(Ngrid, step)
-    t_front = time.perf_counter() - t0
+    t["front"] = time.perf_counter() - t0
+    j0 = Ncut // step
+    shells = list(range(j0, Nmax + 1))
+    run_shells = shells[-1:] if ntri_sample else shells
     t0 = time.perf_counter()
-    d = scipy.fft.fftn(np.where(irk == Nmax, full, 0), workers=cores).real.ravel()
-    t_shell = time.perf_counter() - t0
-    d3 = [d, d * 0.5, d * 0.25]
-    nsum = max(n_tri_sample, 2 * cores)
+    D = {}
+    for j in run_shells:
+        D[j] = scipy.fft.fftn(np.where(irk == j, full, 0).astype(np.complex64),
+                              workers=cores).real.astype(np.float64).ravel()
+        if say:
+            say("  shell %d done (%.1fs)" % (j, time.perf_counter() - t0))
+    t["shells"] = (time.perf_counter() - t0) * len(shells) / len(run_shells)
+    I, J, L, _ = S.triangles(Nmax, Ncut, step)
+    ntri = len(I)
+    if ntri_sample:
+        d = D[shells[-1]]
+        D = {j: d for j in shells}
+    todo = range(min(ntri, ntri_sample)) if ntri_sample else range(ntri)
+
+    def one(k):
+        return np.einsum("i,i,i", D[I[k]], D[J[k]], D[L[k]])
+
+    t0 = time.perf_counter()
     with ThreadPoolExecutor(cores) as ex:
-        t0 = time.perf_counter()
-        list(ex.map(lambda _: np.einsum("i,i,i", d3[0], d3[1], d3[2]), range(nsum)))
-        t_tri = (time.perf_counter() - t0) / nsum
-    ns = Nmax - Ncut // step + 1
-    ntri = len(S.triangles(Nmax, Ncut, step)[0])
-    box_s = t_front + ns * t_shell + ntri * t_tri
-    return {"value": 1.0 / box_s, "unit": "boxes/s", "cores": cores, "kind": "port",
-            "sample": ("oracle restatement with %d cores on the same box: front end %.1fs, 1 of %d "
-                       "shell FFTs %.2fs, %d of %d triangle sums %.4fs each (threaded); box time "
-                       "extrapolated to %.0fs" % (cores, t_front, ns, t_shell, nsum, ntri, t_tri,
-                                                  box_s))}
+        for c, _ in enumerate(ex.map(one, todo)):
+            if say and c % 1000 == 999:
+                say("  %d triangles (%.1fs)" % (c + 1, time.perf_counter() - t0))
+    t["triangles"] = (time.perf_counter() - t0) * ntri / len(todo)
+    t["total"] = t["front"] + t["shells"] + t["triangles"]
+    t["ntri"] = ntri
+    t["nshells"] = len(shells)
+    return t
+
+
+def cpu_baseline(xyz_np, Lbox, Ngrid, step, Ncut, Nmax, n_tri_sample, cores):
+    """Bounded sample of the box on the host (about 10-30 s): the full front
+    end, 1 of the shell FFTs and n_tri_sample triangle sums, extrapolated."""
+    t = cpu_box(xyz_np, Lbox, Ngrid, step, Ncut, Nmax, cores, ntri_sample=n_tri_sample)
+    return {"value": 1.0 / t["total"], "unit": "boxes/s", "cores": cores, "kind": "port",
+            "sample": ("reference-structured oracle path on the same box with %d thread(s) (%s): "
+                       "full front end %.1fs, 1 of %d complex64 shell FFTs (x%d = %.1fs), %d of "
+                       "%d float64 triangle sums (extrapolated %.1fs); box time %.1fs"
+                       % (cores, cpu_model(), t["front"], t["nshells"], t["nshells"], t["shells"],
+                          n_tri_sample, t["ntri"], t["triangles"], t["total"]))}
 
 
 def run_stream(args):
@@ -371,12 +400,66 @@ def run_pmc(args):
     print("pmc program done: %s, Ngrid=%d, Nmax=%d" % (args.precision, args.ngrid, args.nmax))
 
 
+def run_cpu(args):
+    """Full host timings of the reference-structured CPU path, nothing
+    sampled: the plain configuration (1e5 uniform points, seed 1, Lbox 1000,
+    Ngrid 96, step 3, Ncut 3, Nmax 30; P(k) and B(k)) with 1 thread and with
+    all cores, and the single clustered box (1e6 lognormal points, seed 2,
+    Lbox 2500, Ngrid 360, Nmax 40) with all cores; the GPU's FP64 and
+    split-BF16 B(k) of the plain configuration through the Python API beside
+    them.  Progress goes to stderr, one JSON line to stdout."""
+    import torch
+    from oracle import spectrum as S
+    from pyspectrum_amd import mocks
+    from pyspectrum_amd import pyspectrum as P
+
+    def say(m):
+        print(m, file=sys.stderr, flush=True)
+
+    cores = host_cores()
+    res = {"metric": "reference-structured CPU path, full boxes", "cpu_model": cpu_model(),
+           "cores_all": cores}
+    rng = np.random.default_rng(1)
+    xyz1 = rng.uniform(0, 1000.0, size=(3, 100000)).astype(np.float32)
+    for c in (1, cores):
+        say("plain config, %d thread(s)" % c)
+        t0 = time.perf_counter()
+        S.Pk_periodic(xyz1.astype(np.float64), None, 1000.0, 96)
+        tpk = time.perf_counter() - t0
+        t = cpu_box(xyz1.astype(np.float64), 1000.0, 96, 3, 3, 30, c)
+        res["plain_%d_threads" % c] = {"pk_s": round(tpk, 3), "bk_s": round(t["total"], 3),
+                                       "front_s": round(t["front"], 3),
+                                       "shells_s": round(t["shells"], 3),
+                                       "triangles_s": round(t["triangles"], 3)}
+    for prec in ("fp64", "bf16x3"):
+        P.Bk_periodic(xyz1, Lbox=1000.0, Ngrid=96, step=3, Ncut=3, Nmax=30, precision=prec)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            P.Bk_periodic(xyz1, Lbox=1000.0, Ngrid=96, step=3, Ncut=3, Nmax=30, precision=prec)
+            ts.append(time.perf_counter() - t0)
+        res["plain_gpu_%s_call_s" % prec] = round(float(np.median(ts)), 5)
+    say("clustered box, %d threads" % cores)
+    xyz, _ = mocks.lognormal_box(1_000_000, 2500.0, seed=2)
+    xyz2 = xyz.double().cpu().numpy()
+    del xyz
+    t = cpu_box(xyz2, 2500.0, 360, 3, 3, 40, cores, say=say)
+    res["clustered_%d_threads" % cores] = {
+        "bk_s": round(t["total"], 2), "front_s": round(t["front"], 2),
+        "shells_s": round(t["shells"], 2), "triangles_s": round(t["triangles"], 2),
+        "boxes_per_s": 1.0 / t["total"]}
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus == 1 and args.config == "pmc":
         return run_pmc(args)
     if args.gpus == 1 and args.config == "survey":
         return run_survey(args)
+    if args.gpus == 1 and args.config == "cpu":
+        return run_cpu(args)
     if args.gpus == 1 and args.config != "box":
         return run_stream(args)
     if args.gpus != 1:
@@ -398,37 +481,23 @@ def main():
     if args.verbose:
         print("counts ready", file=sys.stderr, flush=True)
 
-    wall, lat = time_steps(box, args.precision, args.steps, args.warmup)
+    wall, lat, kern = time_steps(box, args.precision, args.steps, args.warmup)
     value = args.steps / wall
     out_mixed, p0k_mixed = box.out7.cpu().numpy().copy(), box.p0k.cpu().numpy().copy()
-    wall64, lat64 = time_steps(box, "fp64", args.fp64_steps, 1)
+    wall64, lat64, kern64 = time_steps(box, "fp64", args.fp64_steps, 1)
     v64 = args.fp64_steps / wall64
     err = accuracy(out_mixed, p0k_mixed, box.out7.cpu().numpy(), box.p0k.cpu().numpy(),
                    box.cnt.cpu().numpy())
-    st = stage_times(box, args.precision)
-    st64 = stage_times(box, "fp64")
+    roof = roofline(box, args.precision, kern)
     if args.verbose:
-        print("stages", st, st64, file=sys.stderr, flush=True)
-
-    # roofline of the dominant kernel (the triangle contraction)
-    G = float(N) ** 3
-    ijl = box.ijl
-    npairs = len({(int(i), int(j)) for i, j, _ in ijl})
-    flops = G * (npairs + 2 * box.ntri)
-    t_c = st["contraction"] / 1e3
-    achieved = flops / t_c / 1e12
-    peak = PEAK["bf16_tflops"] if args.precision == "bf16x3" else (
-        PEAK["fp32_tflops"] if args.precision == "fp32" else PEAK["fp64_tflops"])
-    roofline = {"bound": "mfma", "kernel": "k_contract", "achieved": round(achieved, 2),
-                "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                "traffic": contraction_traffic(N, Nmax, args.precision)}
+        print("kernels", kern, kern64, file=sys.stderr, flush=True)
 
     cpu = cpu_all = None
     if not args.no_cpu_baseline:
         xyz_np = xyz.double().cpu().numpy()
-        cpu = cpu_baseline(xyz_np, Lbox, N, step, Ncut, Nmax, args.cpu_sample_triangles)
-        cpu_all = cpu_baseline_all_cores(xyz_np, Lbox, N, step, Ncut, Nmax,
-                                         args.cpu_sample_triangles)
+        cpu = cpu_baseline(xyz_np, Lbox, N, step, Ncut, Nmax, args.cpu_sample_triangles, 1)
+        cpu_all = cpu_baseline(xyz_np, Lbox, N, step, Ncut, Nmax, 4 * args.cpu_sample_triangles,
+                               host_cores())
     out = {
         "metric": "B(k) boxes/s at Ngrid=%d + p50/p99 latency, mixed-precision vs FP64 GPU" % N,
         "value": round(value, 3), "unit": "boxes/s", "n_gpus": 1, "steps": args.steps,
@@ -446,9 +515,9 @@ def main():
         "fp64_p50_ms": round(float(np.percentile(lat64, 50)), 3),
         "mixed_vs_fp64": round(value / v64, 3),
         "error_vs_fp64": err,
-        "stages_ms": {k: round(v, 3) for k, v in st.items()},
-        "stages_fp64_ms": {k: round(v, 3) for k, v in st64.items()},
-        "roofline": roofline,
+        "kernels_ms_per_box": {k: round(v["ms_per_box"], 4) for k, v in kern.items()},
+        "kernels_fp64_ms_per_box": {k: round(v["ms_per_box"], 4) for k, v in kern64.items()},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }

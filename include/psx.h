@@ -63,6 +63,16 @@ int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes);
  * A request loop calls it on an idle context to keep the device memory of
  * all its contexts within a budget.  (No reference counterpart.) */
 int psx_ctx_release_workspace(psx_ctx* ctx);
+/* Kernel timing for measurement tools (no reference counterpart): while
+ * enabled, each launch of the main kernels (k_assign_lines, k_fft_lines,
+ * k_shell_z, k_shell_y, k_shell_x, k_contract, k_xcontract) is bracketed by
+ * a HIP event pair on the context's stream.  Enabling or disabling waits for
+ * the stream and clears what was recorded. */
+int psx_ctx_kernel_timing(psx_ctx* ctx, int enable);
+/* Number of launches of `kernel` recorded since timing was enabled and their
+ * summed device time in ms (waits for the recorded events). */
+int psx_ctx_kernel_time(psx_ctx* ctx, const char* kernel, int64_t* launches /* [host] */,
+                        double* total_ms /* [host] */);
 
 /* Replaces estimator.f:284-512 assign_quad (ia=ib=ic=id=0 branch), called at
  * pyspectrum.py:951 (offset 0) and :819 (offset 0.5*Ngrid).  PCS assignment

@@ -31,6 +31,8 @@ SIGNATURES = {
     "psx_ctx_synchronize": [_p],
     "psx_ctx_workspace_bytes": [_p, _c.POINTER(_i64)],
     "psx_ctx_release_workspace": [_p],
+    "psx_ctx_kernel_timing": [_p, _i],
+    "psx_ctx_kernel_time": [_p, _c.c_char_p, _c.POINTER(_i64), _c.POINTER(_d)],
     "psx_assign_quad": [_p, _p, _p, _i64, _i, _f, _f, _f, _f, _d, _p, _i],
     "psx_fft3d": [_p, _p, _i, _i, _i],
     "psx_fcomb_periodic": [_p, _p, _p, _i, _d, _i],
@@ -144,6 +146,19 @@ class Context:
         """Free the workspace (plans stay) once the context's stream is idle."""
         with self._lock:
             call("psx_ctx_release_workspace", self._h)
+
+    def kernel_timing(self, enable):
+        """Start (clearing earlier records) or stop the event timing of the
+        main kernels' launches on this context's stream."""
+        with self._lock:
+            call("psx_ctx_kernel_timing", self._h, 1 if enable else 0)
+
+    def kernel_time(self, kernel):
+        """(launches, summed device ms) of ``kernel`` since timing was enabled."""
+        n, ms = _i64(), _d()
+        with self._lock:
+            call("psx_ctx_kernel_time", self._h, kernel.encode(), ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
 
     def close(self):
         if self._h:

@@ -552,6 +552,28 @@ static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut
 
 using namespace psx;
 
+psx::TimedLaunch::TimedLaunch(psx_ctx* ctx, const char* name) : ctx_(ctx) {
+  if (!ctx->timing) return;
+  if (ctx->kev_used == ctx->kev.size()) {
+    psx_ctx::KernelEvents e{name, nullptr, nullptr};
+    if (hipEventCreate(&e.start) != hipSuccess) return;
+    if (hipEventCreate(&e.stop) != hipSuccess) {
+      (void)hipEventDestroy(e.start);
+      return;
+    }
+    ctx->kev.push_back(e);
+  }
+  psx_ctx::KernelEvents& e = ctx->kev[ctx->kev_used];
+  e.name = name;
+  if (hipEventRecord(e.start, ctx->stream) != hipSuccess) return;
+  ++ctx->kev_used;
+  stop_ = e.stop;
+}
+
+psx::TimedLaunch::~TimedLaunch() {
+  if (stop_) (void)hipEventRecord(stop_, ctx_->stream);
+}
+
 // ================================================================= C ABI ===
 extern "C" {
 
@@ -583,6 +605,10 @@ int psx_ctx_destroy(psx_ctx* ctx) {
   for (auto& kv : ctx->bufs) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (void* p : ctx->owned) (void)hipFree(p);
   if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
+  for (auto& e : ctx->kev) {
+    (void)hipEventDestroy(e.start);
+    (void)hipEventDestroy(e.stop);
+  }
   delete ctx;
   return PSX_OK;
 }
@@ -614,6 +640,34 @@ int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes) {
   int64_t s = 0;
   for (auto& kv : ctx->bufs) s += (int64_t)kv.second.bytes;
   *bytes = s;
+  return PSX_OK;
+}
+
+int psx_ctx_kernel_timing(psx_ctx* ctx, int enable) {
+  PSX_REQUIRE(ctx, "null ctx");
+  PSX_CTX_LOCK(ctx);
+  PSX_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->timing = enable != 0;
+  ctx->kev_used = 0;
+  return PSX_OK;
+}
+
+int psx_ctx_kernel_time(psx_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms) {
+  PSX_REQUIRE(ctx && kernel && launches && total_ms, "null argument");
+  PSX_CTX_LOCK(ctx);
+  int64_t n = 0;
+  double ms = 0.0;
+  for (size_t i = 0; i < ctx->kev_used; ++i) {
+    const psx_ctx::KernelEvents& e = ctx->kev[i];
+    if (std::strcmp(e.name, kernel) != 0) continue;
+    PSX_HIP(hipEventSynchronize(e.stop));
+    float t = 0.f;
+    PSX_HIP(hipEventElapsedTime(&t, e.start, e.stop));
+    ms += t;
+    ++n;
+  }
+  *launches = n;
+  *total_ms = ms;
   return PSX_OK;
 }
 

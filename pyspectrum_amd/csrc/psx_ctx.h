@@ -130,6 +130,15 @@ struct psx_ctx {
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<psx::ContractPlan>> contract_plans;
   std::map<std::string, psx::DevBuf> bufs;
   std::vector<void*> owned;  // plan allocations
+  // Kernel timing (psx_ctx_kernel_timing): event pairs recorded around the
+  // main launches, reused from one enable to the next.
+  struct KernelEvents {
+    const char* name;
+    hipEvent_t start, stop;
+  };
+  bool timing = false;
+  std::vector<KernelEvents> kev;
+  size_t kev_used = 0;
 };
 
 namespace psx {
@@ -163,6 +172,20 @@ inline int set_lds(K kernel, size_t bytes) {
   }
   return PSX_OK;
 }
+
+// Brackets one launch with a HIP event pair on the context's stream while
+// kernel timing is on; does nothing otherwise.  `name` must be a literal.
+class TimedLaunch {
+ public:
+  TimedLaunch(psx_ctx* ctx, const char* name);
+  ~TimedLaunch();
+  TimedLaunch(const TimedLaunch&) = delete;
+  TimedLaunch& operator=(const TimedLaunch&) = delete;
+
+ private:
+  psx_ctx* ctx_;
+  hipEvent_t stop_ = nullptr;
+};
 
 // stage launchers (defined in the .hip files)
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec);

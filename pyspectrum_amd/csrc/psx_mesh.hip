@@ -357,11 +357,13 @@ static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off, 
     const size_t lds = acc_bytes > fft_bytes ? acc_bytes : fft_bytes;
     auto kern = k_assign_lines<T, NC, true>;
     PSX_TRY(set_lds(kern, lds));
+    TimedLaunch timed(ctx, "k_assign_lines");
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, rec, off, np,
                        p->shape, at, pm, scale, plan_tw<T>(p), grid);
   } else {
     auto kern = k_assign_lines<T, 0, false>;
     PSX_TRY(set_lds(kern, acc_bytes));
+    TimedLaunch timed(ctx, "k_assign_lines");
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, off,
                        np, p->shape, at, pm, scale, plan_tw<T>(p), grid);
   }

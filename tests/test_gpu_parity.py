@@ -226,6 +226,72 @@ def test_bk_raw_matches_oracle(prec, N, step, ncut, nmax):
     assert _rel(p0.cpu().numpy()[j0 - 1:], p0k[j0 - 1:]) < (1e-12 if prec == "fp64" else 1e-6)
 
 
+def test_bk_raw_error_scale_past_nyquist():
+    """Ngrid 128, step 3, Ncut 3, Nmax 40 (the small grids of the batch
+    workload): shells 22-40 lie past the Nyquist sphere and hold only modes
+    near the cube's corners, so their triangle sums cancel by orders of
+    magnitude and a relative error says little.  What rounding can do to a
+    sum is bounded by its absolute sum A_t = sum_x |I_i||I_j||I_l| (FP64
+    shell fields): every path's error must stay a small multiple of its unit
+    round-off times A_t -- split-BF16 (16 significant bits per split operand,
+    fused and staged kernels) and FP32 -- while a lost or wrong term would
+    be of order A_t.  Triangles inside the sphere also meet the 1e-3 spec."""
+    N, step, ncut, nmax = 128, 3, 3, 40
+    _say("bk_raw error scale past Nyquist", N, step, ncut, nmax)
+    rng = np.random.default_rng(13)
+    r = rng.standard_normal((N, N, N))
+    full = np.fft.fftn(r + 0.5 * r ** 2)
+    ijl = L.triangles(step, ncut, nmax)
+    ntri, ns = len(ijl), nmax - ncut // step + 1
+    h64, h32 = _half_of(full, "fp64"), _half_of(full, "fp32")
+
+    def bk_raw(half, prec):
+        b = torch.empty(ntri, dtype=torch.float64, device=DEV)
+        p0 = torch.empty(nmax, dtype=torch.float64, device=DEV)
+        L.call("psx_bk_raw", _h(), L.ptr(half), N, step, ncut, nmax, L.ptr(b), L.ptr(p0),
+               L.PREC[prec])
+        return b.cpu().numpy()
+
+    def contract(d, prec):
+        b = torch.empty(ntri, dtype=torch.float64, device=DEV)
+        L.call("psx_bk_contract", _h(), L.ptr(d), N, step, ncut, nmax, L.ptr(b), L.PREC[prec])
+        return b.cpu().numpy()
+
+    raw = {"fp64": bk_raw(h64, "fp64"), "bf16x3 fused": bk_raw(h32, "bf16x3"),
+           "fp32": bk_raw(h32, "fp32")}
+    d = torch.empty((ns, N, N, N), dtype=torch.float32, device=DEV)
+    L.call("psx_bk_shell_fields", _h(), L.ptr(h32), N, step, ncut, nmax, L.ptr(d), L.PREC["fp32"])
+    raw["bf16x3 staged"] = contract(d, "bf16x3")
+    del d
+    d = torch.empty((ns, N, N, N), dtype=torch.float64, device=DEV)
+    L.call("psx_bk_shell_fields", _h(), L.ptr(h64), N, step, ncut, nmax, L.ptr(d), L.PREC["fp64"])
+    d.abs_()
+    A = contract(d, "fp64")
+    del d
+    torch.cuda.empty_cache()
+    # shells past the grid's corner (|k| > 64 sqrt 3) are empty: A = 0 there
+    # and the staged paths' sums are exactly 0.  The fused path transforms
+    # two shells per complex FFT line, so an empty shell that shares a line
+    # with a non-empty one carries that shell's FP32 round-off (these
+    # triangles have N_tri = 0 and report B = 0).
+    nz = A > 0
+    assert nz.sum() > 1000
+    for name in raw:
+        lim = 0.0 if name in ("fp64", "bf16x3 staged") else 1e-7 * A.max()
+        assert np.all(np.abs(raw[name][~nz]) <= lim), name
+    assert np.all(np.abs(raw["fp64"]) <= A * (1 + 1e-12))
+    inside = (ijl[:, 0] + 0.5) * step <= N / 2
+    ref = raw["fp64"]
+    scale = np.max(np.abs(ref[inside]))
+    for name, bound in (("bf16x3 fused", 2e-5), ("bf16x3 staged", 2e-5), ("fp32", 1e-6)):
+        e = np.where(nz, np.abs(raw[name] - ref) / np.where(nz, A, 1.0), 0.0)
+        e_in = np.max(np.abs(raw[name][inside] - ref[inside])) / scale
+        _say("  %-13s |err|/A max %.2e (inside %.2e, past %.2e); inside rel %.2e"
+             % (name, e.max(), e[inside].max(), e[~inside].max(), e_in))
+        assert e.max() < bound, name
+        assert e_in < TOL["bf16x3" if "bf16" in name else "fp32"], name
+
+
 def test_bk_raw_is_deterministic():
     _say("bk_raw determinism")
     N = 48
