@@ -191,6 +191,37 @@ int psx_bin_power_noalloc(psx_ctx* ctx, const void* half, int ngrid, int step, i
 int psx_pk_pbox_rsd(psx_ctx* ctx, const void* half, int irsd, int lbox, int nbin, int nmu,
                     int ngrid, double* out, int prec);
 
+/* Input side of the serving campaign (no reference counterpart; the
+ * reference takes particle positions from files).  Random numbers are
+ * Philox4x32-10 keyed by `seed`; every output depends only on (seed, index).
+ *
+ * psx_mock_gaussian: Gaussian field on a mesh^3 grid (mesh: an Ngrid the FFT
+ * supports) with P(k) proportional to k^slope exp(-(k/kcut)^2), k in units
+ * of 2 pi / lbox: white noise, forward FFT, amplitude filter, backward FFT,
+ * divided by its sample standard deviation.  g_out: float32 [mesh^3], index
+ * (i0 * mesh + i1) * mesh + i2; sigma_out [host, optional]: the standard
+ * deviation before the division. */
+int psx_mock_gaussian(psx_ctx* ctx, int mesh, double lbox, uint64_t seed, double slope,
+                      double kcut, float* g_out, double* sigma_out /* [host] or NULL */);
+/* psx_mock_sample: np points drawn from the lognormal density exp(g - 1/2)
+ * of a field g (float32 [mesh^3]): each cell with probability proportional
+ * to its weight (prefix sums in a fixed order, first cell whose cumulative
+ * weight reaches u * total), placed uniformly inside it; xyz_out: float32
+ * (3, np), axis a from index i_a, in [0, lbox).  With sigma_v > 0 the
+ * coordinate on axis `los` is moved as psx_apply_rsd does with the velocity
+ * psx_mock_velocities(seed, sigma_v) gives (the fused redshift-space shift,
+ * util.py:54-75). */
+int psx_mock_sample(psx_ctx* ctx, const float* g, int mesh, double lbox, int64_t np,
+                    uint64_t seed, double sigma_v, double rsd_fac, int los, float* xyz_out);
+/* Gaussian velocities: float32 (3, np), sigma * N(0, 1). */
+int psx_mock_velocities(psx_ctx* ctx, int64_t np, uint64_t seed, double sigma, float* v_out);
+/* Replaces util.py:54-75 applyRSD for float32 (3, np) positions and
+ * velocities: on axis `los`, x <- (x + (rsd_fac v + lbox)) % lbox with the
+ * sum formed in float64 and rounded to float32 before the float32 wrap
+ * (numpy's in-place add and remainder); other axes are copied. */
+int psx_apply_rsd(psx_ctx* ctx, const float* xyz, const float* v, int64_t np, double rsd_fac,
+                  double lbox, int los, float* xyz_out);
+
 /* Triangle list in the loop order of _Bk_periodic (pyspectrum.py:415-417). */
 int psx_bk_num_triangles(int step, int ncut, int nmax, int64_t* ntri /* [host] */);
 int psx_bk_triangles(int step, int ncut, int nmax, int32_t* ijl /* [host] [ntri][3] */);

@@ -795,13 +795,18 @@ def ijl_order(i_k, j_k, l_k):
 def apply_rsd(xyz, vxyz, redshift, h, omega0_m, los, Lbox):
     """util.py:54-75: the line-of-sight coordinate becomes
     (x + (f v + Lbox)) % Lbox with f = (1+z)/(100 E(z)), E of flat LCDM with
-    H0 = 100 h; numpy's arithmetic in the arrays' own dtype."""
+    H0 = 100 h, a numpy float64 scalar (the cosmology object's efunc returns
+    one, :67-70).  numpy 2 arithmetic: f v + Lbox is float64 for float32
+    velocities; the in-place add at :73 adds in float64 and rounds to the
+    positions' dtype; the wrap at :74 runs in that dtype (a Python float
+    Lbox takes the array's dtype)."""
     ez = math.sqrt(omega0_m * (1.0 + redshift) ** 3 + (1.0 - omega0_m))
     f = (1 + redshift) / (100 * np.float64(ez))
     out = np.array(xyz, copy=True)
     a = {"x": 0, "y": 1, "z": 2}[los]
-    shifted = out[a] + (f * np.asarray(vxyz)[a] + Lbox)
-    out[a] = shifted % Lbox
+    col = out[a]
+    col += f * np.asarray(vxyz)[a] + Lbox
+    out[a] = col % Lbox
     return out
 
 

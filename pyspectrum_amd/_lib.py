@@ -48,6 +48,10 @@ SIGNATURES = {
     "psx_bin_power": [_p, _p, _i, _i, _i, _d, _p, _i],
     "psx_bin_power_noalloc": [_p, _p, _i, _i, _i, _d, _p, _i],
     "psx_pk_pbox_rsd": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "psx_mock_gaussian": [_p, _i, _d, _c.c_uint64, _d, _d, _p, _c.POINTER(_d)],
+    "psx_mock_sample": [_p, _p, _i, _d, _i64, _c.c_uint64, _d, _d, _i, _p],
+    "psx_mock_velocities": [_p, _i64, _c.c_uint64, _d, _p],
+    "psx_apply_rsd": [_p, _p, _p, _i64, _d, _d, _i, _p],
     "psx_bk_num_triangles": [_i, _i, _i, _c.POINTER(_i64)],
     "psx_bk_triangles": [_i, _i, _i, _p],
     "psx_bk_shell_fields": [_p, _p, _i, _i, _i, _i, _p, _i],

@@ -29,6 +29,13 @@ int launch_bk_finalize(psx_ctx* ctx, const double* braw, const double* p0k, cons
                        const int* tri, int ntri, double G, double nbar, double a3, double b3,
                        double a6, double b6, double* out);
 int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k);
+int launch_mock_gaussian(psx_ctx* ctx, int M, double lbox, uint64_t seed, double slope, double kcut,
+                         float* out, double* sigma_host);
+int launch_mock_sample(psx_ctx* ctx, const float* g, int M, double lbox, int64_t np, uint64_t seed,
+                       double sigma_v, double fac, int los, float* xyz);
+int launch_mock_velocities(psx_ctx* ctx, int64_t np, uint64_t seed, double sigma, float* v);
+int launch_apply_rsd(psx_ctx* ctx, const float* xyz, const float* v, int64_t np, double fac,
+                     double lbox, int los, float* out);
 
 // Serialises the entry points on one context (host-side plan and workspace maps).
 #define PSX_CTX_LOCK(c) std::lock_guard<std::recursive_mutex> psx_ctx_lock_((c)->mu)
@@ -839,6 +846,41 @@ int psx_bin_power_noalloc(psx_ctx* ctx, const void* half, int ngrid, int step, i
   PSX_REQUIRE(step >= 1, "step >= 1 required");
   PSX_TRY(check_ngrid(ngrid));
   return launch_bin_power_noalloc(ctx, half, ngrid, step, nbins, kf, fft_prec(prec), out3);
+}
+
+int psx_mock_gaussian(psx_ctx* ctx, int mesh, double lbox, uint64_t seed, double slope,
+                      double kcut, float* g_out, double* sigma_out) {
+  PSX_REQUIRE(ctx && g_out, "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_TRY(check_ngrid(mesh));
+  PSX_REQUIRE(lbox > 0 && kcut > 0, "lbox and kcut must be positive");
+  return launch_mock_gaussian(ctx, mesh, lbox, seed, slope, kcut, g_out, sigma_out);
+}
+
+int psx_mock_sample(psx_ctx* ctx, const float* g, int mesh, double lbox, int64_t np,
+                    uint64_t seed, double sigma_v, double rsd_fac, int los, float* xyz_out) {
+  PSX_REQUIRE(ctx && g && (xyz_out || np == 0), "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_TRY(check_ngrid(mesh));
+  PSX_REQUIRE(np >= 0 && lbox > 0 && sigma_v >= 0, "np >= 0, lbox > 0 and sigma_v >= 0 required");
+  PSX_REQUIRE(los >= 0 && los <= 2, "los must be 0, 1 or 2");
+  return launch_mock_sample(ctx, g, mesh, lbox, np, seed, sigma_v, rsd_fac, los, xyz_out);
+}
+
+int psx_mock_velocities(psx_ctx* ctx, int64_t np, uint64_t seed, double sigma, float* v_out) {
+  PSX_REQUIRE(ctx && (v_out || np == 0), "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(np >= 0, "negative count");
+  return launch_mock_velocities(ctx, np, seed, sigma, v_out);
+}
+
+int psx_apply_rsd(psx_ctx* ctx, const float* xyz, const float* v, int64_t np, double rsd_fac,
+                  double lbox, int los, float* xyz_out) {
+  PSX_REQUIRE(ctx && ((xyz && v && xyz_out) || np == 0), "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(np >= 0 && lbox > 0, "np >= 0 and lbox > 0 required");
+  PSX_REQUIRE(los >= 0 && los <= 2, "los must be 0, 1 or 2");
+  return launch_apply_rsd(ctx, xyz, v, np, rsd_fac, lbox, los, xyz_out);
 }
 
 int psx_pk_pbox_rsd(psx_ctx* ctx, const void* half, int irsd, int lbox, int nbin, int nmu,

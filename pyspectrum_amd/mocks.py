@@ -1,73 +1,87 @@
-"""Seeded synthetic inputs generated on the GPU: lognormal mocks and the RSD shift.
+"""Seeded synthetic inputs generated on the GPU by the library's kernels:
+lognormal mocks, Gaussian velocities and the redshift-space shift.
 
 Lognormal recipe (the clustered-box workload): a Gaussian field on an M^3
-mesh with P(k) = A k^-1.5 exp(-(k/0.5)^2) (k in h/Mpc), A chosen so the cell
-sigma_G = 1; delta_LN = exp(delta_G - sigma^2/2) - 1; exactly N points drawn
-with probability proportional to 1 + delta_LN per cell (inverse-CDF sampling
-of the cell, i.e. a multinomial draw) and placed uniformly inside the cell.
+mesh with P(k) proportional to k^slope exp(-(k/kcut)^2) (k in h/Mpc),
+normalised to unit cell variance; exactly N points drawn with probability
+proportional to exp(delta_G - 1/2) = 1 + delta_LN per cell (inverse-CDF
+sampling of the cell) and placed uniformly inside it.  PyTorch only holds the
+buffers; the field, the prefix sums, the draws and the shift run in
+psx_mock.hip (Philox4x32-10 random numbers keyed by the seed).
 
-``apply_rsd`` restates applyRSD (pyspectrum/util.py:54-75) for a flat LCDM
-E(z) = sqrt(Om (1+z)^3 + 1 - Om): x_los <- (x_los + v_los (1+z)/(100 E(z)) + L) mod L.
+``apply_rsd`` is applyRSD (pyspectrum/util.py:54-75) for float32 device
+arrays: x_los <- (x_los + (f v_los + Lbox)) % Lbox with f = (1+z)/(100 E(z)),
+the sum rounded to float32 before the wrap, as numpy does in the reference.
 """
-import math
-
 import numpy as np
 import torch
 
+from . import _lib as L
+from . import util
+
+
+def _ctx(device):
+    return L.context(device)
+
+
+def gaussian_field(Lbox, seed, mesh=512, slope=-1.5, kcut=0.5, device=0):
+    """float32 (mesh, mesh, mesh) unit-variance Gaussian field on ``cuda:device``
+    and the standard deviation of the unnormalised draw."""
+    g = torch.empty((mesh, mesh, mesh), dtype=torch.float32, device=torch.device("cuda", device))
+    sig = L._d()
+    _ctx(device).call("psx_mock_gaussian", mesh, float(Lbox), int(seed) & (2 ** 64 - 1),
+                      float(slope), float(kcut), L.ptr(g), L.ctypes.byref(sig))
+    return g, sig.value
+
+
+def sample(g, n, Lbox, seed, sigma_v=0.0, redshift=0.0, omega0_m=0.3, los=2, device=0):
+    """float32 (3, n) points drawn from the lognormal density of field ``g``;
+    with sigma_v > 0, moved to redshift space along ``los`` in the same pass."""
+    mesh = g.shape[0]
+    xyz = torch.empty((3, n), dtype=torch.float32, device=g.device)
+    fac = float(util.rsd_shift(redshift, omega0_m=omega0_m)) if sigma_v > 0 else 0.0
+    _ctx(device).call("psx_mock_sample", L.ptr(g), mesh, float(Lbox), int(n),
+                      int(seed) & (2 ** 64 - 1), float(sigma_v), fac, int(los), L.ptr(xyz))
+    return xyz
+
 
 def lognormal_box(n, Lbox, seed, mesh=512, slope=-1.5, kcut=0.5, device=0):
-    """float32 (3, n) positions on ``cuda:device`` (and the amplitude A used)."""
-    dev = torch.device("cuda", device)
-    g = torch.Generator(device=dev)
-    g.manual_seed(int(seed))
-    M = mesh
-    white = torch.randn((M, M, M), generator=g, device=dev, dtype=torch.float32)
-    wk = torch.fft.rfftn(white)
-    del white
-    kf = 2 * math.pi / Lbox
-    fx = torch.fft.fftfreq(M, d=1.0 / M, device=dev) * kf
-    fz = torch.fft.rfftfreq(M, d=1.0 / M, device=dev) * kf
-    k2 = fx[:, None, None] ** 2 + fx[None, :, None] ** 2 + fz[None, None, :] ** 2
-    k = torch.sqrt(k2)
-    amp = torch.where(k > 0, k.clamp_min(1e-30) ** (slope / 2) * torch.exp(-0.5 * (k / kcut) ** 2),
-                      torch.zeros_like(k))
-    wk *= amp
-    del k, k2, amp
-    dg = torch.fft.irfftn(wk, s=(M, M, M))
-    del wk
-    sig = float(dg.double().std().item())
-    dg /= sig
-    A = 1.0 / sig ** 2  # amplitude (in the units of the unnormalised draw) giving sigma_G = 1
-    w = torch.exp(dg.double() - 0.5).reshape(-1)
-    del dg
-    cdf = torch.cumsum(w, 0)
-    del w
-    u = torch.rand(n, generator=g, device=dev, dtype=torch.float64) * cdf[-1]
-    cell = torch.searchsorted(cdf, u).clamp_max(M ** 3 - 1)
-    del cdf, u
-    ix = cell // (M * M)
-    iy = (cell // M) % M
-    iz = cell % M
-    off = torch.rand((3, n), generator=g, device=dev, dtype=torch.float64)
-    h = Lbox / M
-    xyz = torch.stack([(ix + off[0]) * h, (iy + off[1]) * h, (iz + off[2]) * h])
-    xyz = torch.remainder(xyz, Lbox).to(torch.float32).contiguous()
+    """float32 (3, n) positions on ``cuda:device`` and the amplitude A of the
+    power law (in units of a unit-variance white-noise draw normalised like an
+    inverse real FFT) that gives the cell sigma_G = 1."""
+    g, sig = gaussian_field(Lbox, seed, mesh, slope, kcut, device)
+    xyz = sample(g, n, Lbox, seed, device=device)
+    del g
+    A = (float(mesh) ** 3 / sig) ** 2
     return xyz, A
 
 
+def lognormal_rsd_box(n, Lbox, seed, sigma_v=300.0, redshift=0.5, omega0_m=0.3, los=2, mesh=512,
+                      slope=-1.5, kcut=0.5, device=0):
+    """Redshift-space lognormal box: the points of lognormal_box(seed) moved
+    along ``los`` by Gaussian velocities (gaussian_velocities(seed)), the shift
+    fused into the sampling pass."""
+    g, _ = gaussian_field(Lbox, seed, mesh, slope, kcut, device)
+    return sample(g, n, Lbox, seed, sigma_v, redshift, omega0_m, los, device)
+
+
 def gaussian_velocities(n, sigma, seed, device=0):
-    dev = torch.device("cuda", device)
-    g = torch.Generator(device=dev)
-    g.manual_seed(int(seed) + 7919)
-    return torch.randn((3, n), generator=g, device=dev, dtype=torch.float32) * sigma
+    """float32 (3, n) velocities sigma * N(0, 1)."""
+    v = torch.empty((3, n), dtype=torch.float32, device=torch.device("cuda", device))
+    _ctx(device).call("psx_mock_velocities", int(n), int(seed) & (2 ** 64 - 1), float(sigma),
+                      L.ptr(v))
+    return v
 
 
-def apply_rsd(xyz, vxyz, redshift, Lbox, omega0_m=0.3, los=2):
-    """Redshift-space positions along axis ``los`` (util.py:54-75)."""
-    ez = math.sqrt(omega0_m * (1 + redshift) ** 3 + 1 - omega0_m)
-    fac = (1 + redshift) / (100.0 * ez)
-    out = xyz.clone()
-    out[los] = torch.remainder(xyz[los].double() + fac * vxyz[los].double() + Lbox, Lbox).to(xyz.dtype)
+def apply_rsd(xyz, vxyz, redshift, Lbox, omega0_m=0.3, los=2, h=0.7):
+    """Redshift-space positions along axis ``los`` (util.py:54-75) for float32
+    (3, N) device arrays."""
+    assert xyz.dtype == torch.float32 and vxyz.dtype == torch.float32
+    xyz, vxyz = xyz.contiguous(), vxyz.contiguous()
+    out = torch.empty_like(xyz)
+    fac = float(util.rsd_shift(redshift, h, omega0_m))
+    _ctx(xyz.device.index or 0).call("psx_apply_rsd", L.ptr(xyz), L.ptr(vxyz), xyz.shape[1], fac,
+                                     float(Lbox), int(los), L.ptr(out))
     return out
 
 

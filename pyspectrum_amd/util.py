@@ -131,16 +131,20 @@ _LOS_AXIS = {'x': 0, 'y': 1, 'z': 2}
 
 
 def rsd_shift(redshift, h=0.7, omega0_m=0.3):
-    """Position shift per unit velocity, (1+z)/(100 E(z)) [Mpc/h per km/s]."""
+    """Position shift per unit velocity, (1+z)/(100 E(z)) [Mpc/h per km/s],
+    as a numpy float64 scalar (so arrays it multiplies are promoted to
+    float64, as with the cosmology object's E(z) in util.py:67-70)."""
     ez = FlatLambdaCDM(H0=100. * h, Om0=omega0_m).efunc(redshift)
-    return (1 + redshift) / (100 * ez)
+    return np.float64((1 + redshift) / (100 * ez))
 
 
 def applyRSD(xyz, vxyz, redshift, h=0.7, omega0_m=0.3, LOS=None, Lbox=None):
-    """Redshift-space positions in a periodic box (util.py:54-75): along the
-    line of sight x <- (x + (s v + Lbox)) mod Lbox with s = rsd_shift(z), the
-    reference's order of floating-point operations.  Shapes are asserted as
-    in the reference; LOS and Lbox are required keywords."""
+    """Redshift-space positions in a periodic box (util.py:54-75), with the
+    reference's floating-point steps: the shift s v + Lbox (s = rsd_shift(z),
+    float64) is added to the line-of-sight coordinate in float64 and the sum
+    is rounded to the positions' dtype (numpy's in-place add, util.py:73);
+    the wrap `% Lbox` then runs in that dtype (util.py:74).  Shapes are
+    checked as in the reference; LOS and Lbox are required keywords."""
     pos, vel = np.asarray(xyz), np.asarray(vxyz)
     if pos.shape[0] != 3 or vel.shape[0] != 3:
         raise AssertionError("positions and velocities must be (3, N) arrays")
@@ -150,7 +154,8 @@ def applyRSD(xyz, vxyz, redshift, h=0.7, omega0_m=0.3, LOS=None, Lbox=None):
         raise ValueError("Lbox (box size) is required")
     ax = _LOS_AXIS[LOS]
     out = np.array(pos, copy=True)
-    out[ax] = np.mod(out[ax] + (rsd_shift(redshift, h, omega0_m) * vel[ax] + Lbox), Lbox)
+    moved = (pos[ax] + (rsd_shift(redshift, h, omega0_m) * vel[ax] + Lbox)).astype(out.dtype)
+    out[ax] = np.remainder(moved, Lbox)
     return out
 
 
