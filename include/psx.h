@@ -243,6 +243,30 @@ int psx_bk_contract(psx_ctx* ctx, const void* d, int ngrid, int step, int ncut, 
 int psx_bk_raw(psx_ctx* ctx, const void* half, int ngrid, int step, int ncut, int nmax,
                double* b_raw, double* p0k, int prec);
 
+/* One box of a batch: device float32 positions (3, np) and weights (np, or
+ * NULL), and the per-box scalars of psx_bk_periodic. */
+typedef struct psx_box {
+  const float* xyz;
+  const float* w;
+  int64_t np;
+  double lbox, sumw, wsum_abs, nbar, b3, b6;
+} psx_box;
+
+/* psx_bk_periodic for 1..16 boxes of one shape (ngrid, step, ncut, nmax,
+ * prec) with one launch per stage for all of them: assignment (per-box
+ * counting sorts, tiles of every box in one grid), the backward FFT, fcomb,
+ * the shell passes and the contraction (split-BF16: the fused kernel; FP32
+ * and FP64 run their shell x pass and contraction box by box), the shell
+ * powers and the normalisation.  Outputs per box back to back: out7
+ * [nbox][7][ntri], p0k [nbox][nmax]; half_out (optional, NULL = internal)
+ * [nbox][ngrid][ngrid][ngrid/2+1] complex of the pipeline precision.  Each
+ * box's outputs are bit-identical to psx_bk_periodic on that box.  (The
+ * reference runs boxes one at a time: run/glam/glam.py:98-116.) */
+int psx_bk_periodic_batch(psx_ctx* ctx, int nbox, const psx_box* boxes /* [host] */, int ngrid,
+                          int step, int ncut, int nmax, const double* counts, const int32_t* ijl,
+                          double a3, double a6, double* out7, double* p0k, void* half_out,
+                          int prec);
+
 /* Upper bound of the device workspace (bytes) that psx_bk_periodic on np
  * particles allocates in a context, for admission by memory (the request
  * loop); the caller's own buffers are not included.  (No reference

@@ -22,6 +22,14 @@ _i64 = _c.c_int64
 _d = _c.c_double
 _f = _c.c_float
 
+class PsxBox(ctypes.Structure):
+    """psx_box of include/psx.h: one box of psx_bk_periodic_batch."""
+    _fields_ = [("xyz", ctypes.c_void_p), ("w", ctypes.c_void_p), ("np", ctypes.c_int64),
+                ("lbox", ctypes.c_double), ("sumw", ctypes.c_double),
+                ("wsum_abs", ctypes.c_double), ("nbar", ctypes.c_double),
+                ("b3", ctypes.c_double), ("b6", ctypes.c_double)]
+
+
 # name -> argtypes (all return int status)
 SIGNATURES = {
     "psx_abi_version": [],
@@ -62,6 +70,8 @@ SIGNATURES = {
     "psx_bk_finalize": [_p, _p, _p, _p, _p, _i, _i, _d, _d, _d, _d, _d, _p],
     "psx_bk_periodic": [_p, _p, _p, _i64, _i, _d, _d, _d, _i, _i, _i, _p, _p, _d, _d, _d, _d, _d,
                         _p, _p, _i],
+    "psx_bk_periodic_batch": [_p, _i, _c.POINTER(PsxBox), _i, _i, _i, _i, _p, _p, _d, _d, _p, _p,
+                              _p, _i],
 }
 
 

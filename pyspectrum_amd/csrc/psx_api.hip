@@ -16,7 +16,7 @@ int launch_herm_half(psx_ctx* ctx, const void* F, int in_prec, int64_t sx, int64
                      void* Hh, int N, int prec);
 int launch_reflect(psx_ctx* ctx, const void* Hh, void* F, int N, int prec);
 int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
-                     int prec, double* out3);
+                     int prec, double* out3, int nbox = 1);
 int launch_bin_power_noalloc(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
                              int prec, double* out3);
 int launch_pk_rsd(psx_ctx* ctx, const void* half, int N, int irsd, int lbox, int nbin, int nmu,
@@ -28,7 +28,7 @@ int contract_waves_per_group(int prec, int nlt);
 int launch_bk_finalize(psx_ctx* ctx, const double* braw, const double* p0k, const double* cnt,
                        const int* tri, int ntri, double G, double nbar, double a3, double b3,
                        double a6, double b6, double* out);
-int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k);
+int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k, int nbox = 1);
 int launch_mock_gaussian(psx_ctx* ctx, int M, double lbox, uint64_t seed, double slope, double kcut,
                          float* out, double* sigma_host);
 int launch_mock_sample(psx_ctx* ctx, const float* g, int M, double lbox, int64_t np, uint64_t seed,
@@ -995,6 +995,70 @@ int psx_bk_finalize(psx_ctx* ctx, const double* b_raw, const double* p0k, const 
   if (ntri == 0) return PSX_OK;
   const double G = (double)ngrid * (double)ngrid * (double)ngrid;
   return launch_bk_finalize(ctx, b_raw, p0k, counts, ijl, ntri, G, nbar, a3, b3, a6, b6, out7);
+}
+
+int psx_bk_periodic_batch(psx_ctx* ctx, int nbox, const psx_box* boxes, int ngrid, int step,
+                          int ncut, int nmax, const double* counts, const int32_t* ijl, double a3,
+                          double a6, double* out7, double* p0k, void* half_out, int prec) {
+  PSX_REQUIRE(ctx && boxes && counts && ijl && out7 && p0k, "null argument");
+  PSX_CTX_LOCK(ctx);
+  PSX_REQUIRE(nbox >= 1 && nbox <= PSX_MAX_BATCH, "nbox must be in [1, 16]");
+  PSX_REQUIRE(prec_ok(prec), "unknown precision");
+  PSX_TRY(check_bk_args(ngrid, step, ncut, nmax));
+  const int N = ngrid, fp = fft_prec(prec);
+  const int64_t G = (int64_t)N * N * N, nh = (int64_t)(N / 2 + 1) * N * N;
+  std::vector<const float*> xyz(nbox), w(nbox);
+  std::vector<int64_t> np(nbox);
+  std::vector<PcsMap> pm(nbox);
+  std::vector<double> scale(nbox), sumw(nbox), nbar(nbox), b3(nbox), b6(nbox);
+  for (int b = 0; b < nbox; ++b) {
+    const psx_box& x = boxes[b];
+    PSX_REQUIRE(x.np >= 0 && (x.xyz || x.np == 0) && x.lbox > 0, "bad box");
+    xyz[b] = x.xyz; w[b] = x.w; np[b] = x.np;
+    // the position map of FFT_periodic (pyspectrum.py:920-951)
+    pm[b].kf_ks = (float)((double)N / x.lbox);
+    pm[b].offset = 0.0f;
+    pm[b].lo = 0.0f;
+    pm[b].hi = (float)(x.lbox * (1.0 - 1e-6));
+    scale[b] = ldexp(1.0, fixed_point_exponent(x.wsum_abs));
+    sumw[b] = x.sumw; nbar[b] = x.nbar; b3[b] = x.b3; b6[b] = x.b6;
+  }
+  void *grid, *half = half_out, *braw;
+  PSX_TRY(ctx_buffer(ctx, "fft_grid", (size_t)nbox * G * cbytes(fp), &grid));
+  if (!half) PSX_TRY(ctx_buffer(ctx, "bk_half", (size_t)nbox * nh * cbytes(fp), &half));
+  std::vector<int> tl;
+  triangle_list(step, ncut, nmax, &tl);
+  const int ntri = (int)(tl.size() / 3);
+  PSX_TRY(ctx_buffer(ctx, "bk_braw", (size_t)nbox * std::max(ntri, 1) * sizeof(double), &braw));
+  // FFT_periodic for every box: assignment with the x pass fused, y and z
+  // passes, fcomb
+  PSX_TRY(launch_assign_grid_batch(ctx, nbox, xyz.data(), w.data(), np.data(), pm.data(),
+                                   scale.data(), N, grid, fp, true));
+  PSX_TRY(launch_fft3d_axes_batch(ctx, grid, N, 1, fp, 1, nbox));
+  const FftPlan* ft;
+  PSX_TRY(get_fft_plan(ctx, N, &ft));
+  PSX_TRY(launch_fcomb_batch(ctx, grid, half, N, nbox, sumw.data(), fp, ft->rec, ft->Wf, ft->Wd));
+  // shells and contraction
+  if (xcontract_supported(ctx, N, step, ncut, nmax, prec) == PSX_OK) {
+    PSX_TRY(launch_xcontract(ctx, half, N, step, ncut, nmax, (double*)braw, nbox));
+  } else {
+    const int ns = nmax - ncut / step + 1;
+    void* D;
+    PSX_TRY(ctx_buffer(ctx, "shells", (size_t)ns * G * rbytes(fp), &D));
+    for (int b = 0; b < nbox; ++b) {
+      PSX_TRY(launch_shell_fields(ctx, (const char*)half + (size_t)b * nh * cbytes(fp), N, step,
+                                  ncut, nmax, fp, D));
+      PSX_TRY(launch_contract(ctx, D, N, step, ncut, nmax, prec, (double*)braw + (size_t)b * ntri));
+    }
+  }
+  // shell powers p0k, then units, shot noise and Q
+  void* bins;
+  PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)nbox * 3 * nmax * sizeof(double), &bins));
+  PSX_TRY(launch_bin_power(ctx, half, N, step, nmax, 1.0, fp, (double*)bins, nbox));
+  PSX_TRY(launch_p0k(ctx, (const double*)bins, nmax, ncut / step, p0k, nbox));
+  if (ntri == 0) return PSX_OK;
+  return launch_bk_finalize_batch(ctx, nbox, (const double*)braw, p0k, nmax, counts, ijl, ntri,
+                                  (double)G, nbar.data(), a3, b3.data(), a6, b6.data(), out7);
 }
 
 int psx_bk_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,

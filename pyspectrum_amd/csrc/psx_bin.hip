@@ -30,6 +30,9 @@ template <typename T>
 __global__ __launch_bounds__(BP_THREADS) void k_bin_power(const cplx<T>* __restrict__ Hh, int N,
                                                           int step, int nbins, double kf,
                                                           double* __restrict__ part) {
+  // box blockIdx.y of a batch: its half field and its block partials
+  Hh += (int64_t)blockIdx.y * N * N * (N / 2 + 1);
+  part += (int64_t)blockIdx.y * gridDim.x * 3 * nbins;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* accw = reinterpret_cast<double*>(smem_raw);  // [BP_WAVES][3][nbins]
   for (int i = threadIdx.x; i < BP_WAVES * 3 * nbins; i += BP_THREADS) accw[i] = 0.0;
@@ -91,9 +94,12 @@ __global__ __launch_bounds__(BP_THREADS) void k_bin_power(const cplx<T>* __restr
 }
 
 // out[e] = sum over blocks of part[block][e]: lane l adds blocks l, l+64, ...
-// in order, then a fixed butterfly over the lanes.
+// in order, then a fixed butterfly over the lanes.  Box blockIdx.y of a
+// batch: part [box][nblocks][nent], out [box][nent].
 __global__ __launch_bounds__(64) void k_bin_reduce(const double* __restrict__ part, int nblocks,
                                                    int nent, double* __restrict__ out) {
+  part += (int64_t)blockIdx.y * nblocks * nent;
+  out += (int64_t)blockIdx.y * nent;
   const int e = blockIdx.x, lane = threadIdx.x;
   double s = 0.0;
   for (int b = lane; b < nblocks; b += 64) s += part[(int64_t)b * nent + e];
@@ -116,33 +122,34 @@ static int bin_power_blocks(int N) {
 // k_bin_power + k_bin_reduce with the block partials in `part` (at least
 // bin_power_blocks(N) * 3 * nbins doubles; every entry is written before it
 // is read, so nothing needs clearing).
+// nbox half fields back to back give out3 [nbox][3][nbins].
 static int bin_power_into(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
-                          int prec, double* out3, void* part) {
+                          int prec, double* out3, void* part, int nbox = 1) {
   const size_t lds = (size_t)BP_WAVES * 3 * nbins * sizeof(double);
   const int blocks = bin_power_blocks(N);
   if (prec == PSX_PREC_F64) {
     PSX_TRY(set_lds(k_bin_power<double>, lds));
-    hipLaunchKernelGGL(k_bin_power<double>, dim3(blocks), dim3(BP_THREADS), lds, ctx->stream,
-                       (const cplx<double>*)half, N, step, nbins, kf, (double*)part);
+    hipLaunchKernelGGL(k_bin_power<double>, dim3(blocks, nbox), dim3(BP_THREADS), lds,
+                       ctx->stream, (const cplx<double>*)half, N, step, nbins, kf, (double*)part);
   } else {
     PSX_TRY(set_lds(k_bin_power<float>, lds));
-    hipLaunchKernelGGL(k_bin_power<float>, dim3(blocks), dim3(BP_THREADS), lds, ctx->stream,
-                       (const cplx<float>*)half, N, step, nbins, kf, (double*)part);
+    hipLaunchKernelGGL(k_bin_power<float>, dim3(blocks, nbox), dim3(BP_THREADS), lds,
+                       ctx->stream, (const cplx<float>*)half, N, step, nbins, kf, (double*)part);
   }
   PSX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_bin_reduce, dim3(3 * nbins), dim3(64), 0, ctx->stream, (const double*)part,
-                     blocks, 3 * nbins, out3);
+  hipLaunchKernelGGL(k_bin_reduce, dim3(3 * nbins, nbox), dim3(64), 0, ctx->stream,
+                     (const double*)part, blocks, 3 * nbins, out3);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
 
 int launch_bin_power(psx_ctx* ctx, const void* half, int N, int step, int nbins, double kf,
-                     int prec, double* out3) {
+                     int prec, double* out3, int nbox) {
   PSX_REQUIRE(nbins >= 1 && nbins <= BP_MAX_BINS, "nbins must be in [1, 1024]");
   void* part = nullptr;
-  PSX_TRY(ctx_buffer(ctx, "bin_partials", (size_t)bin_power_blocks(N) * 3 * nbins * sizeof(double),
-                     &part));
-  return bin_power_into(ctx, half, N, step, nbins, kf, prec, out3, part);
+  PSX_TRY(ctx_buffer(ctx, "bin_partials",
+                     (size_t)nbox * bin_power_blocks(N) * 3 * nbins * sizeof(double), &part));
+  return bin_power_into(ctx, half, N, step, nbins, kf, prec, out3, part, nbox);
 }
 
 // ------------------------------------------- bin power without allocation ---
@@ -195,16 +202,19 @@ int launch_bin_power_noalloc(psx_ctx* ctx, const void* half, int N, int step, in
 
 // p0k[j-1] = sum |delta|^2 / Nk[j] for j = max(j0,1)..nmax (NaN if Nk = 0),
 // 0 elsewhere: the shell power of _Bk_periodic (pyspectrum.py:404) by Parseval.
+// Box blockIdx.y of a batch: bins3 [box][3][nmax], p0k [box][nmax].
 __global__ void k_p0k(const double* __restrict__ bins3, int nmax, int j0, double* __restrict__ p0k) {
+  bins3 += (int64_t)blockIdx.y * 3 * nmax;
+  p0k += (int64_t)blockIdx.y * nmax;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nmax) return;
   const int j = i + 1;
   p0k[i] = (j >= j0) ? bins3[nmax + i] / bins3[2 * nmax + i] : 0.0;
 }
 
-int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k) {
-  hipLaunchKernelGGL(k_p0k, dim3((unsigned)cdiv(nmax, 64)), dim3(64), 0, ctx->stream, bins3, nmax,
-                     j0, p0k);
+int launch_p0k(psx_ctx* ctx, const double* bins3, int nmax, int j0, double* p0k, int nbox) {
+  hipLaunchKernelGGL(k_p0k, dim3((unsigned)cdiv(nmax, 64), nbox), dim3(64), 0, ctx->stream, bins3,
+                     nmax, j0, p0k);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }

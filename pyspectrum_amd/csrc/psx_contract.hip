@@ -398,12 +398,14 @@ __global__ __launch_bounds__((ct_waves<MODE, NLT>() * 64), PSX_CT_MINW) void k_c
 // lane*4 + reg of its (row, col) in the MFMA output layout.
 constexpr int RED_SEG = 16;
 
+// Batches: blockIdx.z = box * ngroups + group, partials and red are per box
+// back to back.
 template <typename TP>
 __global__ __launch_bounds__(64 * RED_SEG) void k_contract_reduce(const TP* __restrict__ partials,
                                                                   int64_t nwg, int nslot,
                                                                   double* __restrict__ red) {
   __shared__ double part[RED_SEG][64];
-  const int slot = blockIdx.x, g = blockIdx.z;
+  const int slot = blockIdx.x, g = blockIdx.z;  // g runs over boxes x groups
   const int e = threadIdx.x & 63, seg = threadIdx.x >> 6;
   const int entry = (int)blockIdx.y * 64 + e;
   const int64_t per = (nwg + RED_SEG - 1) / RED_SEG;
@@ -423,25 +425,28 @@ __global__ __launch_bounds__(64 * RED_SEG) void k_contract_reduce(const TP* __re
   }
 }
 
-__global__ void k_contract_gather(const double* __restrict__ red, int nslot,
+// Box blockIdx.y: its reduced blocks and its ntri outputs.
+__global__ void k_contract_gather(const double* __restrict__ red, int nslot, int ngroups,
                                   const int* __restrict__ tri_loc, const int* __restrict__ tri_grp,
                                   int ntri, double* __restrict__ braw) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntri) return;
-  braw[t] = red[(int64_t)tri_grp[t] * nslot * 256 + tri_loc[t]];
+  red += (int64_t)blockIdx.y * ngroups * nslot * 256;
+  braw[(int64_t)blockIdx.y * ntri + t] = red[(int64_t)tri_grp[t] * nslot * 256 + tri_loc[t]];
 }
 
 int launch_contract_reduce(psx_ctx* ctx, const ContractPlan* cp, const double* part, int64_t nwg,
-                           int ngroups, double* braw) {
+                           int ngroups, double* braw, int nbox) {
   const int nslot = cp->nslot;
   void* red = nullptr;
-  PSX_TRY(ctx_buffer(ctx, "contract_red", (size_t)ngroups * nslot * 256 * sizeof(double), &red));
-  hipLaunchKernelGGL(k_contract_reduce<double>, dim3((unsigned)nslot, 4, ngroups),
+  PSX_TRY(ctx_buffer(ctx, "contract_red", (size_t)nbox * ngroups * nslot * 256 * sizeof(double),
+                     &red));
+  hipLaunchKernelGGL(k_contract_reduce<double>, dim3((unsigned)nslot, 4, nbox * ngroups),
                      dim3(64 * RED_SEG), 0, ctx->stream, part, nwg, nslot, (double*)red);
   PSX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_contract_gather, dim3((unsigned)cdiv(cp->ntri, 128)), dim3(128), 0,
-                     ctx->stream, (const double*)red, nslot, cp->d_tri_loc, cp->d_tri_grp,
-                     cp->ntri, braw);
+  hipLaunchKernelGGL(k_contract_gather, dim3((unsigned)cdiv(cp->ntri, 128), nbox), dim3(128), 0,
+                     ctx->stream, (const double*)red, nslot, ngroups, cp->d_tri_loc,
+                     cp->d_tri_grp, cp->ntri, braw);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
@@ -533,15 +538,25 @@ int launch_contract(psx_ctx* ctx, const void* D, int N, int step, int ncut, int 
 // tri: [ntri][3] shell indices (i, j, l); p0k[j-1]; cnt[t] = the normalisation
 // sum_x I_i I_j I_l.  Zero-count triangles produce zeros before the unit
 // conversion, as in the reference.
+// Box blockIdx.y of a batch: braw [box][ntri], p0k [box][p0k_stride],
+// out [box][7][ntri], its own nbar and kf^3, kf^6.
 struct BkConst {
-  double G, nbar, a3, b3, a6, b6;
+  double G, a3, a6;
+  double nbar[PSX_MAX_BATCH], b3[PSX_MAX_BATCH], b6[PSX_MAX_BATCH];
 };
 
 __global__ void k_bk_finalize(const double* __restrict__ braw, const double* __restrict__ p0k,
                               const double* __restrict__ cnt, const int* __restrict__ tri, int ntri,
-                              BkConst c, double* __restrict__ out /* [7][ntri] */) {
+                              int p0k_stride, BkConst bc, double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntri) return;
+  const int box = blockIdx.y;
+  braw += (int64_t)box * ntri;
+  p0k += (int64_t)box * p0k_stride;
+  out += (int64_t)box * 7 * ntri;
+  struct {
+    double G, nbar, a3, b3, a6, b6;
+  } c = {bc.G, bc.nbar[box], bc.a3, bc.b3[box], bc.a6, bc.b6[box]};
   const int i = tri[3 * t], j = tri[3 * t + 1], l = tri[3 * t + 2];
   double fac = 1.0;
   if (i == j && j == l) fac = 6.0;
@@ -569,14 +584,25 @@ __global__ void k_bk_finalize(const double* __restrict__ braw, const double* __r
   out[6 * (int64_t)ntri + t] = sn;
 }
 
+int launch_bk_finalize_batch(psx_ctx* ctx, int nbox, const double* braw, const double* p0k,
+                             int p0k_stride, const double* cnt, const int* tri, int ntri, double G,
+                             const double* nbar, double a3, const double* b3, double a6,
+                             const double* b6, double* out) {
+  PSX_REQUIRE(nbox >= 1 && nbox <= PSX_MAX_BATCH, "batch of 1 to 16 boxes");
+  BkConst c{};
+  c.G = G; c.a3 = a3; c.a6 = a6;
+  for (int b = 0; b < nbox; ++b) { c.nbar[b] = nbar[b]; c.b3[b] = b3[b]; c.b6[b] = b6[b]; }
+  hipLaunchKernelGGL(k_bk_finalize, dim3((unsigned)cdiv(ntri, 128), nbox), dim3(128), 0,
+                     ctx->stream, braw, p0k, cnt, tri, ntri, p0k_stride, c, out);
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
 int launch_bk_finalize(psx_ctx* ctx, const double* braw, const double* p0k, const double* cnt,
                        const int* tri, int ntri, double G, double nbar, double a3, double b3,
                        double a6, double b6, double* out) {
-  BkConst c{G, nbar, a3, b3, a6, b6};
-  hipLaunchKernelGGL(k_bk_finalize, dim3((unsigned)cdiv(ntri, 128)), dim3(128), 0, ctx->stream,
-                     braw, p0k, cnt, tri, ntri, c, out);
-  PSX_CHECK_LAUNCH();
-  return PSX_OK;
+  return launch_bk_finalize_batch(ctx, 1, braw, p0k, 0, cnt, tri, ntri, G, &nbar, a3, &b3, a6, &b6,
+                                  out);
 }
 
 }  // namespace psx

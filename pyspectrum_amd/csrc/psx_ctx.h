@@ -12,6 +12,10 @@
 
 struct psx_ctx;
 
+// Boxes of one shape that one batched launch per stage processes
+// (psx_bk_periodic_batch); per-box scalars travel as kernel arguments.
+#define PSX_MAX_BATCH 16
+
 namespace psx {
 
 // Device buffer owned by the context (freed at ctx destroy).
@@ -191,6 +195,9 @@ class TimedLaunch {
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec);
 // line passes along axes first_axis..2 only (0 = x, 1 = y, 2 = z)
 int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis);
+// the same on nbox grids back to back (box stride N^3)
+int launch_fft3d_axes_batch(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis,
+                            int nbox);
 // Position -> grid map of the assignment: rx = kf_ks*x + 1 + offset (float32,
 // estimator.f:302-304) after clipping x to [lo, hi].
 struct PcsMap {
@@ -200,19 +207,33 @@ struct PcsMap {
 // fft_x: also transform every x line (backward, unnormalised) before the store.
 int launch_assign_grid(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
                        PcsMap pm, double scale, void* grid, int prec, bool fft_x);
+// nbox boxes into nbox grids back to back (box stride N^3).
+int launch_assign_grid_batch(psx_ctx* ctx, int nbox, const float* const* xyz,
+                             const float* const* w, const int64_t* np, const PcsMap* pm,
+                             const double* scale, int N, void* grid, int prec, bool fft_x);
+// fcomb of nbox grids into nbox half fields (sumw[b] per box).
+int launch_fcomb_batch(psx_ctx* ctx, const void* F, void* Hh, int N, int nbox, const double* sumw,
+                       int prec, const void* rec, const float* Wf, const double* Wd);
+int launch_bk_finalize_batch(psx_ctx* ctx, int nbox, const double* braw, const double* p0k,
+                             int p0k_stride, const double* cnt, const int* tri, int ntri, double G,
+                             const double* nbar, double a3, const double* b3, double a6,
+                             const double* b6, double* out);
 int fixed_point_exponent(double wsum_abs);
 int launch_shell_fields(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                         int prec, void* dout);
 // z and y passes of the FP32 shell fields only (the x pass is fused into the
 // contraction): returns the plan and the intermediate T2.
+// nbox same-shape half fields back to back (box stride N*N*(N/2+1)); T2
+// holds nbox intermediates (box stride sp->t_elems).
 int launch_shell_zy(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
-                    const ShellPlan** sp_out, const void** t2_out);
+                    const ShellPlan** sp_out, const void** t2_out, int nbox = 1);
 // Split-BF16 contraction with the shell x pass fused (PSX_ERR_UNSUPPORTED when
 // the shape is outside what the fused kernel covers; the caller then runs the
 // separate stages).
 int xcontract_supported(psx_ctx* ctx, int N, int step, int ncut, int nmax, int prec);
+// nbox half fields back to back -> braw [nbox][ntri]
 int launch_xcontract(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
-                     double* braw);
+                     double* braw, int nbox = 1);
 int launch_fill_unit(psx_ctx* ctx, double* half, int64_t n);
 int launch_survey_catalogue(psx_ctx* ctx, const double* radecz, const double* nb, const double* w,
                             int64_t n, const double* dist_nodes, const double* dist_slopes, int nseg,
@@ -225,8 +246,9 @@ int launch_five_delta2_2(psx_ctx* ctx, const void* g, void* xx, const void* xy, 
 int launch_build_quad(psx_ctx* ctx, const void* d1, void* d2, int irsd, int N, int fp);
 int launch_quad_weights(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ia, int ib,
                         int ic, int id, float* out);
-// Fixed-order float64 reduction of contraction partials [ngroups][nwg][nslot][256]
-// and the gather of each triangle's sum (reference loop order).
+// Fixed-order float64 reduction of contraction partials
+// [nbox][ngroups][nwg][nslot][256] and the gather of each triangle's sum
+// (reference loop order) into braw [nbox][ntri].
 int launch_contract_reduce(psx_ctx* ctx, const ContractPlan* cp, const double* part, int64_t nwg,
-                           int ngroups, double* braw);
+                           int ngroups, double* braw, int nbox = 1);
 }  // namespace psx

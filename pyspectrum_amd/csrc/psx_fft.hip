@@ -20,12 +20,13 @@
 namespace psx {
 
 // ------------------------------------------------------------------ passes --
-// Batched line FFT.  Line (o, c) starts at o*Os + c*Cs, element n at +n*Es.
-// A tile holds tl consecutive inner lines c of one outer index o.
+// Batched line FFT.  Line (o, c) of box blockIdx.y starts at
+// blockIdx.y*Bs + o*Os + c*Cs, element n at +n*Es.  A tile holds tl
+// consecutive inner lines c of one outer index o.
 template <typename T, int S, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
     const cplx<T>* in, cplx<T>* out, FftShape sh, const cplx<T>* __restrict__ tw, int64_t Os,
-    int64_t Cs, int64_t Es, int n_inner) {
+    int64_t Cs, int64_t Es, int n_inner, int64_t Bs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int N = n_of<NC>(sh);
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
   const int64_t o = blockIdx.x / ntile;
   const int c0 = (int)(blockIdx.x - o * ntile) * tl;
   const int nl = min(tl, n_inner - c0);
-  const int64_t base = o * Os + (int64_t)c0 * Cs;
+  const int64_t base = (int64_t)blockIdx.y * Bs + o * Os + (int64_t)c0 * Cs;
   // offsets within a tile fit in 32 bits for N <= 1024; tl is a constant for
   // compile-time N, so the index divisions below are multiplications
   const int cs = (int)Cs, es = (int)Es;
@@ -73,7 +74,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_lines(
 }
 
 template <typename T, int S, int NC>
-static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out, int axis) {
+static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out, int axis,
+                      int nbox) {
   const int N = p->shape.N;
   const int64_t NN = (int64_t)N * N;
   int64_t Os, Cs, Es;
@@ -87,8 +89,8 @@ static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>
   PSX_TRY(set_lds(kern, lds));
   {
     TimedLaunch timed(ctx, "k_fft_lines");
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(FFT_THREADS), lds, ctx->stream, in, out,
-                       p->shape, plan_tw<T>(p), Os, Cs, Es, N);
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks, (unsigned)nbox), dim3(FFT_THREADS), lds,
+                       ctx->stream, in, out, p->shape, plan_tw<T>(p), Os, Cs, Es, N, NN * N);
   }
   PSX_CHECK_LAUNCH();
   return PSX_OK;
@@ -96,32 +98,38 @@ static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>
 
 #define PSX_CASE_N(n)                                                                        \
   case n:                                                                                    \
-    return run_pass_n<T, S, n>(ctx, p, in, out, axis);
+    return run_pass_n<T, S, n>(ctx, p, in, out, axis, nbox);
 
 template <typename T, int S>
-static int run_pass(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out, int axis) {
+static int run_pass(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>* out, int axis,
+                    int nbox) {
   switch (p->shape.N) {
     PSX_FFT_SIZES(PSX_CASE_N)
-    default: return run_pass_n<T, S, 0>(ctx, p, in, out, axis);
+    default: return run_pass_n<T, S, 0>(ctx, p, in, out, axis, nbox);
   }
 }
 #undef PSX_CASE_N
 
 template <typename T, int S>
-static int fft3d_t(psx_ctx* ctx, cplx<T>* g, int N, int first_axis) {
+static int fft3d_t(psx_ctx* ctx, cplx<T>* g, int N, int first_axis, int nbox) {
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
-  for (int axis = first_axis; axis < 3; ++axis) PSX_TRY((run_pass<T, S>(ctx, p, g, g, axis)));
+  for (int axis = first_axis; axis < 3; ++axis) PSX_TRY((run_pass<T, S>(ctx, p, g, g, axis, nbox)));
   return PSX_OK;
 }
 
-int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis) {
+int launch_fft3d_axes_batch(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis,
+                            int nbox) {
   if (prec == PSX_PREC_F64) {
-    return sign > 0 ? fft3d_t<double, 1>(ctx, (cplx<double>*)grid, N, first_axis)
-                    : fft3d_t<double, -1>(ctx, (cplx<double>*)grid, N, first_axis);
+    return sign > 0 ? fft3d_t<double, 1>(ctx, (cplx<double>*)grid, N, first_axis, nbox)
+                    : fft3d_t<double, -1>(ctx, (cplx<double>*)grid, N, first_axis, nbox);
   }
-  return sign > 0 ? fft3d_t<float, 1>(ctx, (cplx<float>*)grid, N, first_axis)
-                  : fft3d_t<float, -1>(ctx, (cplx<float>*)grid, N, first_axis);
+  return sign > 0 ? fft3d_t<float, 1>(ctx, (cplx<float>*)grid, N, first_axis, nbox)
+                  : fft3d_t<float, -1>(ctx, (cplx<float>*)grid, N, first_axis, nbox);
+}
+
+int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis) {
+  return launch_fft3d_axes_batch(ctx, grid, N, sign, prec, first_axis, 1);
 }
 
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec) {
@@ -155,7 +163,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
                                                          cplx<T>* __restrict__ T1,
                                                          const Tile3* __restrict__ tiles,
                                                          ShellDev sd, FftShape sh,
-                                                         const cplx<T>* __restrict__ tw) {
+                                                         const cplx<T>* __restrict__ tw,
+                                                         int64_t hbs, int64_t tbs) {
+  // box blockIdx.y of a batch: its half field and intermediate
+  H += (int64_t)blockIdx.y * hbs;
+  T1 += (int64_t)blockIdx.y * tbs;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int N = n_of<NC>(sh), Nh = sd.Nh, lstride = N + 1;
@@ -201,7 +213,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
                                                          cplx<T>* __restrict__ T2,
                                                          const Tile3* __restrict__ tiles,
                                                          ShellDev sd, FftShape sh,
-                                                         const cplx<T>* __restrict__ tw) {
+                                                         const cplx<T>* __restrict__ tw,
+                                                         int64_t tbs) {
+  T1 += (int64_t)blockIdx.y * tbs;
+  T2 += (int64_t)blockIdx.y * tbs;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int N = n_of<NC>(sh), lstride = N + 1;
@@ -315,7 +330,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_x(const cplx<T>* __restri
 
 template <typename T, int NC>
 static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, const cplx<T>* H,
-                          int step, cplx<T>* t1, cplx<T>* t2, T* dout, bool do_x = true) {
+                          int step, cplx<T>* t1, cplx<T>* t2, T* dout, bool do_x = true,
+                          int nbox = 1) {
+  const int64_t hbs = (int64_t)p->shape.N * p->shape.N * (p->shape.N / 2 + 1);
   const int N = p->shape.N;
   ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
   const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
@@ -325,8 +342,8 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
     PSX_TRY(set_lds(k_shell_z<T, NC>, ldsz));
     {
       TimedLaunch timed(ctx, "k_shell_z");
-      hipLaunchKernelGGL((k_shell_z<T, NC>), dim3(sp->nz_tiles), dim3(FFT_THREADS), ldsz, ctx->stream,
-                         H, t1, sp->d_ztiles, sd, p->shape, tw);
+      hipLaunchKernelGGL((k_shell_z<T, NC>), dim3(sp->nz_tiles, nbox), dim3(FFT_THREADS), ldsz,
+                         ctx->stream, H, t1, sp->d_ztiles, sd, p->shape, tw, hbs, sp->t_elems);
     }
     PSX_CHECK_LAUNCH();
   }
@@ -334,8 +351,9 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
     PSX_TRY(set_lds(k_shell_y<T, NC>, ldsz));
     {
       TimedLaunch timed(ctx, "k_shell_y");
-      hipLaunchKernelGGL((k_shell_y<T, NC>), dim3(sp->ny_tiles), dim3(FFT_THREADS), ldsz, ctx->stream,
-                         (const cplx<T>*)t1, t2, sp->d_ytiles, sd, p->shape, tw);
+      hipLaunchKernelGGL((k_shell_y<T, NC>), dim3(sp->ny_tiles, nbox), dim3(FFT_THREADS), ldsz,
+                         ctx->stream, (const cplx<T>*)t1, t2, sp->d_ytiles, sd, p->shape, tw,
+                         sp->t_elems);
     }
     PSX_CHECK_LAUNCH();
   }
@@ -380,10 +398,10 @@ static int shell_fields_t(psx_ctx* ctx, const cplx<T>* H, int N, int step, int n
 #undef PSX_CASE_N
 
 #define PSX_CASE_N(n) \
-  case n: return shell_fields_n<float, n>(ctx, p, sp, H, step, t1, t2, nullptr, false);
+  case n: return shell_fields_n<float, n>(ctx, p, sp, H, step, t1, t2, nullptr, false, nbox);
 
 int launch_shell_zy(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
-                    const ShellPlan** sp_out, const void** t2_out) {
+                    const ShellPlan** sp_out, const void** t2_out, int nbox) {
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
   int nc = 0;
@@ -393,8 +411,8 @@ int launch_shell_zy(psx_ctx* ctx, const void* half, int N, int step, int ncut, i
   const ShellPlan* sp;
   PSX_TRY(get_shell_plan(ctx, N, step, ncut, nmax, TileCfg<float>::LINES, shell_x_pairs<float>(nc), &sp));
   void *v1 = nullptr, *v2 = nullptr;
-  PSX_TRY(ctx_buffer(ctx, "shell_t1_f32", (size_t)sp->t_elems * sizeof(cplx<float>), &v1));
-  PSX_TRY(ctx_buffer(ctx, "shell_t2_f32", (size_t)sp->t_elems * sizeof(cplx<float>), &v2));
+  PSX_TRY(ctx_buffer(ctx, "shell_t1_f32", (size_t)nbox * sp->t_elems * sizeof(cplx<float>), &v1));
+  PSX_TRY(ctx_buffer(ctx, "shell_t2_f32", (size_t)nbox * sp->t_elems * sizeof(cplx<float>), &v2));
   *sp_out = sp;
   *t2_out = v2;
   const cplx<float>* H = (const cplx<float>*)half;
@@ -402,7 +420,7 @@ int launch_shell_zy(psx_ctx* ctx, const void* half, int N, int step, int ncut, i
   cplx<float>* t2 = (cplx<float>*)v2;
   switch (N) {
     PSX_FFT_SIZES(PSX_CASE_N)
-    default: return shell_fields_n<float, 0>(ctx, p, sp, H, step, t1, t2, nullptr, false);
+    default: return shell_fields_n<float, 0>(ctx, p, sp, H, step, t1, t2, nullptr, false, nbox);
   }
 }
 #undef PSX_CASE_N

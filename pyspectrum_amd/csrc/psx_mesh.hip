@@ -114,21 +114,41 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0u;
 }
 
-__global__ void k_tile_count(const float* __restrict__ xyz, int64_t np, int N, PcsMap pm,
-                             AssignTiling at, unsigned* __restrict__ count,
-                             unsigned* __restrict__ key, unsigned* __restrict__ rank) {
+// The boxes of a batched assignment (kernel argument): box b's particles,
+// its position map and fixed-point scale, and the offset of its particles
+// in the sort buffers.  Tile counts and offsets are per box back to back.
+struct AssignBoxes {
+  const float* xyz[PSX_MAX_BATCH];
+  const float* w[PSX_MAX_BATCH];
+  int64_t np[PSX_MAX_BATCH];
+  int64_t poff[PSX_MAX_BATCH];
+  PcsMap pm[PSX_MAX_BATCH];
+  double scale[PSX_MAX_BATCH];
+};
+
+__global__ void k_tile_count(const AssignBoxes bs, int N, AssignTiling at,
+                             unsigned* __restrict__ count, unsigned* __restrict__ key,
+                             unsigned* __restrict__ rank) {
+  const int b = blockIdx.y;
+  const int64_t np = bs.np[b];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
+  const float* xyz = bs.xyz[b];
+  const PcsMap pm = bs.pm[b];
+  count += (int64_t)b * at.nty * at.ntz;
   const unsigned k = tile_key(clip_pos(xyz[np + i], pm), clip_pos(xyz[2 * np + i], pm), N, pm, at);
-  key[i] = k;
-  rank[i] = atomicAdd(count + k, 1u);
+  key[bs.poff[b] + i] = k;
+  rank[bs.poff[b] + i] = atomicAdd(count + k, 1u);
 }
 
 // Exclusive scan of n counts into off[0..n] (off[n] = total), one workgroup:
 // thread t sums a contiguous segment, the segment sums are scanned in LDS.
+// Box blockIdx.y: count [box][n], off [box][n + 1].
 constexpr int SCAN_THREADS = 1024;
 __global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const unsigned* __restrict__ count,
                                                             int n, unsigned* __restrict__ off) {
+  count += (int64_t)blockIdx.y * n;
+  off += (int64_t)blockIdx.y * (n + 1);
   __shared__ unsigned part[SCAN_THREADS];
   const int t = threadIdx.x;
   const int seg = (n + SCAN_THREADS - 1) / SCAN_THREADS;
@@ -151,13 +171,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const unsigned* __re
   if (t == SCAN_THREADS - 1) off[n] = part[t];
 }
 
-__global__ void k_tile_scatter(const float* __restrict__ xyz, const float* __restrict__ w,
-                               const unsigned* __restrict__ key, const unsigned* __restrict__ rank,
-                               const unsigned* __restrict__ off, int64_t np, PcsMap pm,
-                               float4* __restrict__ rec) {
+__global__ void k_tile_scatter(const AssignBoxes bs, const unsigned* __restrict__ key,
+                               const unsigned* __restrict__ rank, const unsigned* __restrict__ off,
+                               int ntiles, float4* __restrict__ rec) {
+  const int b = blockIdx.y;
+  const int64_t np = bs.np[b];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  rec[(int64_t)off[key[i]] + rank[i]] =
+  const float* xyz = bs.xyz[b];
+  const float* w = bs.w[b];
+  const PcsMap pm = bs.pm[b];
+  const int64_t p = bs.poff[b] + i;
+  off += (int64_t)b * (ntiles + 1);
+  rec[bs.poff[b] + (int64_t)off[key[p]] + rank[p]] =
       make_float4(clip_pos(xyz[i], pm), clip_pos(xyz[np + i], pm), clip_pos(xyz[2 * np + i], pm),
                   w ? w[i] : 1.0f);
 }
@@ -185,11 +211,19 @@ __device__ __forceinline__ int wrap1(int c, int N) {
 
 // LDS: int64 accumulators acc[line][x][grid] (16 B per cell); with FFTX the
 // same memory then holds the complex lines (row stride N+1) and the twiddles.
+// Box blockIdx.y of a batch: its sorted records (from bs.poff), tile
+// offsets [box][ntiles + 1] and grid out [box][N^3].
 template <typename T, int NC, bool FFTX>
 __global__ __launch_bounds__(FFT_THREADS) void k_assign_lines(
-    const float4* __restrict__ rec, const unsigned* __restrict__ tile_off, int64_t np, FftShape sh,
-    AssignTiling at, PcsMap pm, double scale, const cplx<T>* __restrict__ tw,
-    cplx<T>* __restrict__ out) {
+    const float4* __restrict__ rec, const unsigned* __restrict__ tile_off, const AssignBoxes bs,
+    FftShape sh, AssignTiling at, const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ out) {
+  const int box = blockIdx.y;
+  const int64_t np = bs.np[box];
+  const PcsMap pm = bs.pm[box];
+  const double scale = bs.scale[box];
+  rec += bs.poff[box];
+  tile_off += (int64_t)box * (at.nty * at.ntz + 1);
+  out += (int64_t)box * n_of<NC>(sh) * n_of<NC>(sh) * n_of<NC>(sh);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem_raw);
   __shared__ int src[64];
@@ -345,27 +379,28 @@ int fixed_point_exponent(double wsum_abs) {
 }
 
 template <typename T, int NC>
-static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off, int64_t np, int N,
-                          AssignTiling at, PcsMap pm, double scale, cplx<T>* grid, bool fft_x) {
+static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off,
+                          const AssignBoxes& bs, int nbox, int N, AssignTiling at, cplx<T>* grid,
+                          bool fft_x) {
   const FftPlan* p;
   PSX_TRY(get_fft_plan(ctx, N, &p));
   const size_t nl = (size_t)at.ty * at.tz;
   const size_t acc_bytes = 16 * nl * N;
-  const unsigned nblocks = (unsigned)(at.nty * at.ntz);
+  const dim3 blocks((unsigned)(at.nty * at.ntz), (unsigned)nbox);
   if (fft_x) {
     const size_t fft_bytes = sizeof(cplx<T>) * (nl * (N + 1) + N);
     const size_t lds = acc_bytes > fft_bytes ? acc_bytes : fft_bytes;
     auto kern = k_assign_lines<T, NC, true>;
     PSX_TRY(set_lds(kern, lds));
     TimedLaunch timed(ctx, "k_assign_lines");
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), lds, ctx->stream, rec, off, np,
-                       p->shape, at, pm, scale, plan_tw<T>(p), grid);
+    hipLaunchKernelGGL(kern, blocks, dim3(FFT_THREADS), lds, ctx->stream, rec, off, bs, p->shape,
+                       at, plan_tw<T>(p), grid);
   } else {
     auto kern = k_assign_lines<T, 0, false>;
     PSX_TRY(set_lds(kern, acc_bytes));
     TimedLaunch timed(ctx, "k_assign_lines");
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, off,
-                       np, p->shape, at, pm, scale, plan_tw<T>(p), grid);
+    hipLaunchKernelGGL(kern, blocks, dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, off, bs,
+                       p->shape, at, plan_tw<T>(p), grid);
   }
   PSX_CHECK_LAUNCH();
   return PSX_OK;
@@ -373,54 +408,75 @@ static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off, 
 
 #define PSX_CASE_N(n)                                                                         \
   case n:                                                                                     \
-    return assign_lines_n<T, n>(ctx, rec, off, np, N, at, pm, scale, grid, fft_x);
+    return assign_lines_n<T, n>(ctx, rec, o, bs, nbox, N, at, grid, fft_x);
 
+// nbox boxes (particles xyz[b], weights w[b] or null, counts np[b], maps
+// pm[b], fixed-point scales scale[b]) into nbox grids back to back.
 template <typename T>
-static int assign_grid_t(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
-                         PcsMap pm, double scale, cplx<T>* grid, bool fft_x) {
-  PSX_REQUIRE(np < (int64_t)1 << 32, "more than 2^32 - 1 particles");
+static int assign_grid_t(psx_ctx* ctx, int nbox, const float* const* xyz, const float* const* w,
+                         const int64_t* np, const PcsMap* pm, const double* scale, int N,
+                         cplx<T>* grid, bool fft_x) {
+  PSX_REQUIRE(nbox >= 1 && nbox <= PSX_MAX_BATCH, "batch of 1 to 16 boxes");
   const AssignTiling at = assign_tiling<T>(N);
+  AssignBoxes bs{};
+  int64_t total = 0, maxnp = 0;
+  for (int b = 0; b < nbox; ++b) {
+    PSX_REQUIRE(np[b] >= 0 && np[b] < (int64_t)1 << 32, "more than 2^32 - 1 particles");
+    bs.xyz[b] = xyz[b];
+    bs.w[b] = w ? w[b] : nullptr;
+    bs.np[b] = np[b];
+    bs.poff[b] = total;
+    bs.pm[b] = pm[b];
+    bs.scale[b] = scale[b];
+    total += np[b];
+    maxnp = std::max(maxnp, np[b]);
+  }
+  const int64_t ntiles = (int64_t)at.nty * at.ntz;
   const float4* rec = nullptr;
-  const unsigned* off = nullptr;
-  if (np > 0) {
-    // counting sort by assignment tile into (x, y, z, w) records
-    const int64_t ntiles = (int64_t)at.nty * at.ntz;
-    unsigned *count, *key, *rank, *o;
+  unsigned* o = nullptr;
+  PSX_TRY(ctx_buffer(ctx, "tile_off", (size_t)nbox * (ntiles + 1) * 4, (void**)&o));
+  if (total > 0) {
+    // counting sort by assignment tile into (x, y, z, w) records, per box
+    unsigned *count, *key, *rank;
     float4* r;
-    PSX_TRY(ctx_buffer(ctx, "tile_count", (size_t)ntiles * 4, (void**)&count));
-    PSX_TRY(ctx_buffer(ctx, "tile_off", (size_t)(ntiles + 1) * 4, (void**)&o));
-    PSX_TRY(ctx_buffer(ctx, "particle_tile", (size_t)np * 4, (void**)&key));
-    PSX_TRY(ctx_buffer(ctx, "particle_rank", (size_t)np * 4, (void**)&rank));
-    PSX_TRY(ctx_buffer(ctx, "sorted_particles", (size_t)np * sizeof(float4), (void**)&r));
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)cdiv(ntiles, 256)), dim3(256), 0, ctx->stream,
-                       count, ntiles);
+    PSX_TRY(ctx_buffer(ctx, "tile_count", (size_t)nbox * ntiles * 4, (void**)&count));
+    PSX_TRY(ctx_buffer(ctx, "particle_tile", (size_t)total * 4, (void**)&key));
+    PSX_TRY(ctx_buffer(ctx, "particle_rank", (size_t)total * 4, (void**)&rank));
+    PSX_TRY(ctx_buffer(ctx, "sorted_particles", (size_t)total * sizeof(float4), (void**)&r));
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)cdiv(nbox * ntiles, 256)), dim3(256), 0,
+                       ctx->stream, count, nbox * ntiles);
     PSX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_tile_count, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream, xyz,
-                       np, N, pm, at, count, key, rank);
+    const dim3 pgrid((unsigned)cdiv(maxnp, 256), (unsigned)nbox);
+    hipLaunchKernelGGL(k_tile_count, pgrid, dim3(256), 0, ctx->stream, bs, N, at, count, key, rank);
     PSX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream,
+    hipLaunchKernelGGL(k_tile_scan, dim3(1, nbox), dim3(SCAN_THREADS), 0, ctx->stream,
                        (const unsigned*)count, (int)ntiles, o);
     PSX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_tile_scatter, dim3((unsigned)cdiv(np, 256)), dim3(256), 0, ctx->stream,
-                       xyz, w, (const unsigned*)key, (const unsigned*)rank, (const unsigned*)o, np,
-                       pm, r);
+    hipLaunchKernelGGL(k_tile_scatter, pgrid, dim3(256), 0, ctx->stream, bs, (const unsigned*)key,
+                       (const unsigned*)rank, (const unsigned*)o, (int)ntiles, r);
     PSX_CHECK_LAUNCH();
     rec = r;
-    off = o;
   }
-  if (!fft_x) return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, pm, scale, grid, false);
+  // boxes without particles read no records (their np is 0)
+  if (!fft_x) return assign_lines_n<T, 0>(ctx, rec, o, bs, nbox, N, at, grid, false);
   switch (N) {
     PSX_FFT_SIZES(PSX_CASE_N)
-    default: return assign_lines_n<T, 0>(ctx, rec, off, np, N, at, pm, scale, grid, true);
+    default: return assign_lines_n<T, 0>(ctx, rec, o, bs, nbox, N, at, grid, true);
   }
 }
 #undef PSX_CASE_N
 
+int launch_assign_grid_batch(psx_ctx* ctx, int nbox, const float* const* xyz,
+                             const float* const* w, const int64_t* np, const PcsMap* pm,
+                             const double* scale, int N, void* grid, int prec, bool fft_x) {
+  if (prec == PSX_PREC_F64)
+    return assign_grid_t<double>(ctx, nbox, xyz, w, np, pm, scale, N, (cplx<double>*)grid, fft_x);
+  return assign_grid_t<float>(ctx, nbox, xyz, w, np, pm, scale, N, (cplx<float>*)grid, fft_x);
+}
+
 int launch_assign_grid(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
                        PcsMap pm, double scale, void* grid, int prec, bool fft_x) {
-  if (prec == PSX_PREC_F64)
-    return assign_grid_t<double>(ctx, xyz, w, np, N, pm, scale, (cplx<double>*)grid, fft_x);
-  return assign_grid_t<float>(ctx, xyz, w, np, N, pm, scale, (cplx<float>*)grid, fft_x);
+  return launch_assign_grid_batch(ctx, 1, &xyz, &w, &np, &pm, &scale, N, grid, prec, fft_x);
 }
 
 // Unit field delta = 1 (the shell indicators of _counts_Bk123), complex double.
@@ -464,12 +520,21 @@ __device__ __forceinline__ void slot_put(cplx<T> (&m)[8], int s, cplx<T> v) {
   for (int k = 0; k < 8; ++k) if (s == k) m[k] = v;
 }
 
+// Normalisations of the boxes of a batch (kernel argument).
+template <typename T> struct FcombCoef {
+  T cf[PSX_MAX_BATCH];
+};
+
+// Box blockIdx.y of a batch: grid F [box][N^3], half field Hh [box][N^2 (N/2+1)].
 template <typename T>
 __global__ __launch_bounds__(256) void k_fcomb(const cplx<T>* __restrict__ F, cplx<T>* __restrict__ Hh, int N,
-                                               T cf, const c64* __restrict__ rec,
+                                               FcombCoef<T> coef, const c64* __restrict__ rec,
                                                const float* __restrict__ Wf,
                                                const double* __restrict__ Wd) {
   const int Nh = N / 2 + 1;
+  F += (int64_t)blockIdx.y * N * N * N;
+  Hh += (int64_t)blockIdx.y * N * N * Nh;
+  const T cf = coef.cf[blockIdx.y];
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (int64_t)Nh * Nh * Nh) return;
   const int ix = (int)(id % Nh);
@@ -526,23 +591,32 @@ __global__ __launch_bounds__(256) void k_fcomb(const cplx<T>* __restrict__ F, cp
   }
 }
 
-int launch_fcomb(psx_ctx* ctx, const void* F, void* Hh, int N, double sumw, int prec,
-                 const void* rec, const float* Wf, const double* Wd) {
+// nbox grids back to back -> nbox half fields back to back.
+int launch_fcomb_batch(psx_ctx* ctx, const void* F, void* Hh, int N, int nbox, const double* sumw,
+                       int prec, const void* rec, const float* Wf, const double* Wd) {
+  PSX_REQUIRE(nbox >= 1 && nbox <= PSX_MAX_BATCH, "batch of 1 to 16 boxes");
   const int64_t Nh = N / 2 + 1;
   const int64_t n = Nh * Nh * Nh;
-  // cf = 1/(6^3 * 4 * N) with N = sum w as float32 (estimator.f:615)
-  const float sumw32 = (float)sumw;
+  const dim3 grid((unsigned)cdiv(n, 256), nbox);
   if (prec == PSX_PREC_F64) {
-    const double cf = 1.0 / (216.0 * 4.0 * (double)sumw32);
-    hipLaunchKernelGGL(k_fcomb<double>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream,
-                       (const cplx<double>*)F, (cplx<double>*)Hh, N, cf, (const c64*)rec, Wf, Wd);
+    FcombCoef<double> c{};
+    // cf = 1/(6^3 * 4 * N) with N = sum w as float32 (estimator.f:615)
+    for (int b = 0; b < nbox; ++b) c.cf[b] = 1.0 / (216.0 * 4.0 * (double)(float)sumw[b]);
+    hipLaunchKernelGGL(k_fcomb<double>, grid, dim3(256), 0, ctx->stream, (const cplx<double>*)F,
+                       (cplx<double>*)Hh, N, c, (const c64*)rec, Wf, Wd);
   } else {
-    const float cf = 1.0f / (864.0f * sumw32);
-    hipLaunchKernelGGL(k_fcomb<float>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream,
-                       (const cplx<float>*)F, (cplx<float>*)Hh, N, cf, (const c64*)rec, Wf, Wd);
+    FcombCoef<float> c{};
+    for (int b = 0; b < nbox; ++b) c.cf[b] = 1.0f / (864.0f * (float)sumw[b]);
+    hipLaunchKernelGGL(k_fcomb<float>, grid, dim3(256), 0, ctx->stream, (const cplx<float>*)F,
+                       (cplx<float>*)Hh, N, c, (const c64*)rec, Wf, Wd);
   }
   PSX_CHECK_LAUNCH();
   return PSX_OK;
+}
+
+int launch_fcomb(psx_ctx* ctx, const void* F, void* Hh, int N, double sumw, int prec,
+                 const void* rec, const float* Wf, const double* Wd) {
+  return launch_fcomb_batch(ctx, F, Hh, N, 1, &sumw, prec, rec, Wf, Wd);
 }
 
 // ------------------------------------------------- half <-> full fields ----

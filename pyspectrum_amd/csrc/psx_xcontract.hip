@@ -144,7 +144,7 @@ __device__ __forceinline__ void xc_stages(cplx<float>* lines, int nlines,
 
 // T2: per shell s, rows (z*N + y) of kxw_s complex values at offset off[s];
 // meta[4s + 1] = rmax_s, meta[4s + 2] = kxw_s.  items: (line, kx) pairs.
-// partials: [gridDim.x][nslot][256] float64.
+// partials: [box][gridDim.x][nslot][256] float64.
 template <int N, int NLT, int NLINES>
 __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
     const cplx<float>* __restrict__ T2, const int64_t* __restrict__ off,
@@ -152,7 +152,11 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
     const int* __restrict__ line_kxl, const cplx<float>* __restrict__ tw,
     const int* __restrict__ wave_pt, const int* __restrict__ wave_mask,
     const int* __restrict__ wave_slot, const int* __restrict__ pair_s, int rows_per_block,
-    int nslot, double* __restrict__ partials, long long* __restrict__ phase_clocks) {
+    int nslot, double* __restrict__ partials, long long* __restrict__ phase_clocks,
+    int64_t t2_bs) {
+  // box blockIdx.y of a batch: its intermediate and its partials
+  T2 += (int64_t)blockIdx.y * t2_bs;
+  partials += (int64_t)blockIdx.y * gridDim.x * nslot * 256;
   typedef XcGeom<N> Gm;
   constexpr int KC = Gm::KC, L = Gm::L, LDWH = Gm::LDWH;
   constexpr int Nh = N / 2 + 1;
@@ -440,7 +444,7 @@ int xcontract_supported(psx_ctx* ctx, int N, int step, int ncut, int nmax, int p
 
 template <int N, int NLT, int NLINES>
 static int xcontract_t(psx_ctx* ctx, const ContractPlan* cp, const ShellPlan* sp,
-                       const cplx<float>* t2, const FftPlan* fp, double* braw) {
+                       const cplx<float>* t2, const FftPlan* fp, double* braw, int nbox) {
   const int ns = cp->ns;
   PSX_REQUIRE(sp->nxlines <= NLINES && sp->nxitems <= XC_THREADS * XC_NPF,
               "fused contraction: shell plan outside the kernel's limits");
@@ -459,7 +463,8 @@ static int xcontract_t(psx_ctx* ctx, const ContractPlan* cp, const ShellPlan* sp
   const int rows_per_block = (int)cdiv(nrows, slots);
   const int nwg = (int)cdiv(nrows, rows_per_block);
   void* part = nullptr;
-  PSX_TRY(ctx_buffer(ctx, "contract_partials", (size_t)nwg * cp->nslot * 256 * sizeof(double), &part));
+  PSX_TRY(ctx_buffer(ctx, "contract_partials",
+                     (size_t)nbox * nwg * cp->nslot * 256 * sizeof(double), &part));
   long long* clocks = nullptr;
 #ifdef PSX_XC_PHASE_CLOCKS
   PSX_TRY(ctx_buffer(ctx, "xc_phase_clocks", (size_t)nwg * XC_WAVES * 8 * sizeof(long long),
@@ -467,10 +472,11 @@ static int xcontract_t(psx_ctx* ctx, const ContractPlan* cp, const ShellPlan* sp
 #endif
   {
     TimedLaunch timed(ctx, "k_xcontract");
-    hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(XC_THREADS), lds, ctx->stream, t2, sp->d_off,
-                       sp->d_meta, ns, sp->d_xitems, sp->nxitems, sp->d_xline_kxl,
-                       plan_tw<float>(fp), cp->d_wave_pt, cp->d_wave_mask, cp->d_wave_slot,
-                       cp->d_pair_s, rows_per_block, cp->nslot, (double*)part, clocks);
+    hipLaunchKernelGGL(kern, dim3((unsigned)nwg, (unsigned)nbox), dim3(XC_THREADS), lds,
+                       ctx->stream, t2, sp->d_off, sp->d_meta, ns, sp->d_xitems, sp->nxitems,
+                       sp->d_xline_kxl, plan_tw<float>(fp), cp->d_wave_pt, cp->d_wave_mask,
+                       cp->d_wave_slot, cp->d_pair_s, rows_per_block, cp->nslot, (double*)part,
+                       clocks, sp->t_elems);
   }
   PSX_CHECK_LAUNCH();
 #ifdef PSX_XC_PHASE_CLOCKS
@@ -488,25 +494,25 @@ static int xcontract_t(psx_ctx* ctx, const ContractPlan* cp, const ShellPlan* sp
     }
   }
 #endif
-  return launch_contract_reduce(ctx, cp, (const double*)part, nwg, 1, braw);
+  return launch_contract_reduce(ctx, cp, (const double*)part, nwg, 1, braw, nbox);
 }
 
 template <int N>
 static int xcontract_n(psx_ctx* ctx, const ContractPlan* cp, const ShellPlan* sp,
-                       const cplx<float>* t2, const FftPlan* fp, double* braw) {
+                       const cplx<float>* t2, const FftPlan* fp, double* braw, int nbox) {
   const int nl = xc_nlines_ct(cp->ns);
   switch (cp->nlt) {
-    case 1: return xcontract_t<N, 1, 8>(ctx, cp, sp, t2, fp, braw);
-    case 2: return xcontract_t<N, 2, 16>(ctx, cp, sp, t2, fp, braw);
+    case 1: return xcontract_t<N, 1, 8>(ctx, cp, sp, t2, fp, braw, nbox);
+    case 2: return xcontract_t<N, 2, 16>(ctx, cp, sp, t2, fp, braw, nbox);
     case 3:
-      return nl <= 20 ? xcontract_t<N, 3, 20>(ctx, cp, sp, t2, fp, braw)
-                      : xcontract_t<N, 3, 24>(ctx, cp, sp, t2, fp, braw);
+      return nl <= 20 ? xcontract_t<N, 3, 20>(ctx, cp, sp, t2, fp, braw, nbox)
+                      : xcontract_t<N, 3, 24>(ctx, cp, sp, t2, fp, braw, nbox);
     default: return set_error(PSX_ERR_UNSUPPORTED, "fused contraction: at most 48 shells");
   }
 }
 
 int launch_xcontract(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
-                     double* braw) {
+                     double* braw, int nbox) {
   PSX_TRY(xcontract_supported(ctx, N, step, ncut, nmax, PSX_PREC_BF16X3));
   const ContractPlan* cp;
   PSX_TRY(get_contract_plan(ctx, PSX_PREC_BF16X3 | CT_LAYOUT_FUSED, step, ncut, nmax, &cp));
@@ -514,9 +520,9 @@ int launch_xcontract(psx_ctx* ctx, const void* half, int N, int step, int ncut, 
   PSX_TRY(get_fft_plan(ctx, N, &fp));
   const ShellPlan* sp;
   const void* t2;
-  PSX_TRY(launch_shell_zy(ctx, half, N, step, ncut, nmax, &sp, &t2));
+  PSX_TRY(launch_shell_zy(ctx, half, N, step, ncut, nmax, &sp, &t2, nbox));
   switch (N) {
-#define PSX_CASE_N(n) case n: return xcontract_n<n>(ctx, cp, sp, (const cplx<float>*)t2, fp, braw);
+#define PSX_CASE_N(n) case n: return xcontract_n<n>(ctx, cp, sp, (const cplx<float>*)t2, fp, braw, nbox);
     PSX_XC_SIZES(PSX_CASE_N)
 #undef PSX_CASE_N
     default: return set_error(PSX_ERR_UNSUPPORTED, "fused contraction: unsupported Ngrid");

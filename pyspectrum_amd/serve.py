@@ -11,6 +11,10 @@ and B(k) (Bk_periodic, pyspectrum.py:285).  The loop
   request does not fit, the workspace of idle slots is released first;
 * runs admitted requests concurrently, one library context and one
   default-priority stream per slot, so their kernels overlap on the device;
+* with ``max_batch`` > 1, batches requests continuously: a request admitted
+  to a slot takes along the requests of the same shape (Ngrid, step, Ncut,
+  Nmax, precision) that have arrived by then, up to ``max_batch``, and the
+  slot runs them with one launch per stage (psx_bk_periodic_batch);
 * computes the half field once per request and reuses it for P_l and B;
 * records arrival, start and completion times (latency = completion - arrival).
 
@@ -118,18 +122,41 @@ class Slot:
                    L.ptr(ijl_d), ntri, N, float(nbar), (2 * np.pi) ** 3, kf ** 3, (2 * np.pi) ** 6,
                    kf ** 6, L.ptr(out7))
             fin = out7.cpu().numpy()[:, :ntri]   # synchronises this stream
-        k, p0, p2, p4, nk = pk[:5]
-        norm = (2. * np.pi) ** 3
-        return {"k": k, "p0k": p0 * norm - 1. / nbar, "p2k": p2 * norm, "p4k": p4 * norm,
-                "counts": nk, "b123": fin[3], "q123": fin[4], "p0k1": fin[0], "p0k2": fin[1],
-                "p0k3": fin[2], "b_counts": fin[5], "b123_sn": fin[6]}
+        return _result(pk, nbar, fin)
+
+    def run_batch(self, reqs):
+        """Requests of one shape as one batch: B(k) of all boxes with one launch
+        per stage (psx_bk_periodic_batch), then P_l of each box from its half
+        field.  Each result is bit-identical to ``run`` on that request."""
+        from .batch import run_batch
+        r0 = reqs[0]
+        N = r0.Ngrid
+        prec = L.PREC[r0.precision]
+        with torch.cuda.stream(self.stream):
+            out7, _, half, sums, ijl, _, keep = run_batch(
+                self.ctx, [r.xyz for r in reqs], None, [r.Lbox for r in reqs], N, r0.step,
+                r0.Ncut, r0.Nmax, r0.precision, self.device)
+            pks = [P._pk_pbox_rsd(half[b], 2, r.Lbox, N // 2, r.Nmubin, N, prec, self.device,
+                                  ctx=self.ctx) for b, r in enumerate(reqs)]
+            fin = out7.cpu().numpy()[:, :, :len(ijl)]   # synchronises this stream
+            del keep
+        return [_result(pk, nbar, fin[b]) for b, (pk, (_, _, nbar, _)) in enumerate(zip(pks, sums))]
+
+
+def _result(pk, nbar, fin):
+    k, p0, p2, p4, nk = pk[:5]
+    norm = (2. * np.pi) ** 3
+    return {"k": k, "p0k": p0 * norm - 1. / nbar, "p2k": p2 * norm, "p4k": p4 * norm,
+            "counts": nk, "b123": fin[3], "q123": fin[4], "p0k1": fin[0], "p0k2": fin[1],
+            "p0k3": fin[2], "b_counts": fin[5], "b123_sn": fin[6]}
 
 
 class Server:
     """Admission by slots and HBM budget; requests run concurrently on slots."""
 
-    def __init__(self, nslots=4, hbm_budget=None, device=0, use_graphs=False):
+    def __init__(self, nslots=4, hbm_budget=None, device=0, use_graphs=False, max_batch=1):
         self.device = device
+        self.max_batch = max(1, min(int(max_batch), 16))
         free, total = torch.cuda.mem_get_info(device)
         self.budget = int(hbm_budget if hbm_budget is not None else 0.8 * free)
         self.use_graphs = use_graphs
@@ -220,29 +247,54 @@ class Server:
         threads = []
         t0 = time.perf_counter()
 
-        def work(req, s, need):
+        def work(group, s, need):
             # a failing request (bad input, out of memory) records its error
             # and frees its slot; the others are unaffected
             try:
-                req.t_start = time.perf_counter() - t0
-                req.result = self.slots[s].run(req)
+                for req in group:
+                    req.t_start = time.perf_counter() - t0
+                if len(group) == 1:
+                    results = [self.slots[s].run(group[0])]
+                else:
+                    results = self.slots[s].run_batch(group)
+                for req, res in zip(group, results):
+                    req.result = res
             except Exception as e:
-                req.error = "%s: %s" % (type(e).__name__, e)
+                for req in group:
+                    req.error = "%s: %s" % (type(e).__name__, e)
             finally:
-                req.t_done = time.perf_counter() - t0
+                for req in group:
+                    req.t_done = time.perf_counter() - t0
                 self._release(s, need)
 
-        for req in reqs:
+        def shape(r):
+            return (r.Ngrid, r.step, r.Ncut, r.Nmax, r.precision)
+
+        taken = [False] * len(reqs)
+        for i, req in enumerate(reqs):
+            if taken[i]:
+                continue
             delay = req.arrival - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
+            # continuous batching: requests of the same shape that have
+            # arrived by now join this one (up to max_batch)
+            group, now = [i], time.perf_counter() - t0
+            for j in range(i + 1, len(reqs)):
+                if len(group) == self.max_batch or reqs[j].arrival > now:
+                    break
+                if not taken[j] and shape(reqs[j]) == shape(req):
+                    group.append(j)
+            for j in group:
+                taken[j] = True
+            grp = [reqs[j] for j in group]
             try:
-                need = footprint_bytes(req.Ngrid, req.xyz.shape[1], req.Nmax, req.step, req.Ncut,
-                                       req.precision, req.Nmubin, self.device)
+                need = sum(footprint_bytes(r.Ngrid, r.xyz.shape[1], r.Nmax, r.step, r.Ncut,
+                                           r.precision, r.Nmubin, self.device) for r in grp)
             except (L.PsxError, ValueError):
                 need = 0  # an unsupported request fails on its slot and records the error
             s = self._admit(need)
-            th = threading.Thread(target=work, args=(req, s, need), daemon=True)
+            th = threading.Thread(target=work, args=(grp, s, need), daemon=True)
             th.start()
             threads.append(th)
         for th in threads:
