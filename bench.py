@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=16,
                     help="serve/batch: same-shape requests batched per slot (1 = no batching)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="batch: only the batched measurement (for kernel traces)")
     ap.add_argument("--batch-slots", type=int, default=2,
                     help="batch: slots of the batched measurement (a 16-box batch at Ngrid 360 "
                          "holds about 100 GB of workspace)")
@@ -98,8 +100,8 @@ class Box:
 
 
 # Kernels the library brackets with HIP events while kernel timing is on.
-KERNELS = ("k_assign_lines", "k_fft_lines", "k_shell_z", "k_shell_y", "k_shell_x", "k_contract",
-           "k_xcontract")
+KERNELS = ("k_assign_lines", "k_fft_lines", "k_fft_zcomb", "k_shell_z", "k_shell_y", "k_shell_x",
+           "k_contract", "k_xcontract")
 
 
 def time_steps(box, prec, steps, warmup):
@@ -326,15 +328,17 @@ def run_stream(args):
         # batch, up to --max-batch boxes); then one box per slot, eager and
         # with one captured B(k) graph per slot and Ngrid (built before the clock)
         rate, st, wall, _ = measure(False, args.max_batch, args.batch_slots)
-        rate_1, st_1, _, _ = measure(False)
-        rate_g, st_g, wall_g, tg = measure(True)
-        extra = {"max_batch": args.max_batch, "batch_slots": args.batch_slots,
-                 "unbatched_slots": args.slots,
-                 "unbatched_boxes_per_s": round(rate_1, 3),
-                 "unbatched_p50_ms": round(st_1["p50_ms"], 2),
-                 "unbatched_p99_ms": round(st_1["p99_ms"], 2),
-                 "graph_boxes_per_s": round(rate_g, 3), "graph_p50_ms": round(st_g["p50_ms"], 2),
-                 "graph_p99_ms": round(st_g["p99_ms"], 2), "graph_build_s": round(tg, 3)}
+        extra = {"max_batch": args.max_batch, "batch_slots": args.batch_slots}
+        if not args.batched_only:
+            rate_1, st_1, _, _ = measure(False)
+            rate_g, st_g, wall_g, tg = measure(True)
+            extra.update({"unbatched_slots": args.slots,
+                          "unbatched_boxes_per_s": round(rate_1, 3),
+                          "unbatched_p50_ms": round(st_1["p50_ms"], 2),
+                          "unbatched_p99_ms": round(st_1["p99_ms"], 2),
+                          "graph_boxes_per_s": round(rate_g, 3),
+                          "graph_p50_ms": round(st_g["p50_ms"], 2),
+                          "graph_p99_ms": round(st_g["p99_ms"], 2), "graph_build_s": round(tg, 3)})
     else:
         rate, st, wall, _ = measure(False, args.max_batch)
         rate_1, st_1, _, _ = measure(False)

@@ -65,9 +65,9 @@ int psx_ctx_workspace_bytes(psx_ctx* ctx, int64_t* bytes);
 int psx_ctx_release_workspace(psx_ctx* ctx);
 /* Kernel timing for measurement tools (no reference counterpart): while
  * enabled, each launch of the main kernels (k_assign_lines, k_fft_lines,
- * k_shell_z, k_shell_y, k_shell_x, k_contract, k_xcontract) is bracketed by
- * a HIP event pair on the context's stream.  Enabling or disabling waits for
- * the stream and clears what was recorded. */
+ * k_fft_zcomb, k_shell_z, k_shell_y, k_shell_x, k_contract, k_xcontract) is
+ * bracketed by a HIP event pair on the context's stream.  Enabling or
+ * disabling waits for the stream and clears what was recorded. */
 int psx_ctx_kernel_timing(psx_ctx* ctx, int enable);
 /* Number of launches of `kernel` recorded since timing was enabled and their
  * summed device time in ms (waits for the recorded events). */
@@ -99,7 +99,9 @@ int psx_fft3d(psx_ctx* ctx, void* grid, int ngrid, int sign, int prec);
 int psx_fcomb_periodic(psx_ctx* ctx, const void* grid, void* half_out, int ngrid, double sumw,
                        int prec);
 
-/* Replaces FFT_periodic (pyspectrum.py:909-959): assign + FFT + fcomb. */
+/* Replaces FFT_periodic (pyspectrum.py:909-959): assign + FFT + fcomb, as
+ * three kernels: the assignment with the x pass of the backward FFT fused,
+ * the y pass, and the z pass with fcomb fused (writes the half field). */
 int psx_fft_periodic(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int ngrid,
                      double lbox, double sumw, double wsum_abs, void* half_out, int prec);
 

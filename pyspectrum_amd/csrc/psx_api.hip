@@ -511,13 +511,11 @@ static int mesh_to_half(psx_ctx* ctx, const float* xyz, const float* w, int64_t 
   void* grid;
   PSX_TRY(ctx_buffer(ctx, "fft_grid", G * cbytes(fp), &grid));
   const double scale = ldexp(1.0, fixed_point_exponent(wsum_abs));
-  // assignment with the x pass of the backward FFT fused, then the y and z passes
+  // assignment with the x pass of the backward FFT fused, the y pass, then
+  // the z pass with fcomb as its epilogue (writes the half field)
   PSX_TRY(launch_assign_grid(ctx, xyz, w, np, N, pm, scale, grid, fp, true));
-  PSX_TRY(launch_fft3d_axes(ctx, grid, N, 1, fp, 1));
-  const FftPlan* ft;
-  PSX_TRY(get_fft_plan(ctx, N, &ft));
-  PSX_TRY(launch_fcomb(ctx, grid, half, N, sumw, fp, ft->rec, ft->Wf, ft->Wd));
-  return PSX_OK;
+  PSX_TRY(launch_fft_axis_batch(ctx, grid, N, 1, fp, 1, 1));
+  return launch_fft_zcomb_batch(ctx, grid, half, N, 1, &sumw, fp);
 }
 
 static int fft_periodic_impl(psx_ctx* ctx, const float* xyz, const float* w, int64_t np, int N,
@@ -1030,14 +1028,12 @@ int psx_bk_periodic_batch(psx_ctx* ctx, int nbox, const psx_box* boxes, int ngri
   triangle_list(step, ncut, nmax, &tl);
   const int ntri = (int)(tl.size() / 3);
   PSX_TRY(ctx_buffer(ctx, "bk_braw", (size_t)nbox * std::max(ntri, 1) * sizeof(double), &braw));
-  // FFT_periodic for every box: assignment with the x pass fused, y and z
-  // passes, fcomb
+  // FFT_periodic for every box: assignment with the x pass fused, the y
+  // pass, the z pass with fcomb fused
   PSX_TRY(launch_assign_grid_batch(ctx, nbox, xyz.data(), w.data(), np.data(), pm.data(),
                                    scale.data(), N, grid, fp, true));
-  PSX_TRY(launch_fft3d_axes_batch(ctx, grid, N, 1, fp, 1, nbox));
-  const FftPlan* ft;
-  PSX_TRY(get_fft_plan(ctx, N, &ft));
-  PSX_TRY(launch_fcomb_batch(ctx, grid, half, N, nbox, sumw.data(), fp, ft->rec, ft->Wf, ft->Wd));
+  PSX_TRY(launch_fft_axis_batch(ctx, grid, N, 1, fp, 1, nbox));
+  PSX_TRY(launch_fft_zcomb_batch(ctx, grid, half, N, nbox, sumw.data(), fp));
   // shells and contraction
   if (xcontract_supported(ctx, N, step, ncut, nmax, prec) == PSX_OK) {
     PSX_TRY(launch_xcontract(ctx, half, N, step, ncut, nmax, (double*)braw, nbox));

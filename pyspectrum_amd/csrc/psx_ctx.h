@@ -198,6 +198,12 @@ int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int f
 // the same on nbox grids back to back (box stride N^3)
 int launch_fft3d_axes_batch(psx_ctx* ctx, void* grid, int N, int sign, int prec, int first_axis,
                             int nbox);
+// one axis of it
+int launch_fft_axis_batch(psx_ctx* ctx, void* grid, int N, int sign, int prec, int axis, int nbox);
+// The last (z) pass of the backward grid FFT fused with fcomb: nbox grids
+// after their x and y passes -> nbox half fields (sumw[b] per box).
+int launch_fft_zcomb_batch(psx_ctx* ctx, const void* F, void* Hh, int N, int nbox,
+                           const double* sumw, int prec);
 // Position -> grid map of the assignment: rx = kf_ks*x + 1 + offset (float32,
 // estimator.f:302-304) after clipping x to [lo, hi].
 struct PcsMap {

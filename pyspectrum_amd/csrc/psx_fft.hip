@@ -132,6 +132,19 @@ int launch_fft3d_axes(psx_ctx* ctx, void* grid, int N, int sign, int prec, int f
   return launch_fft3d_axes_batch(ctx, grid, N, sign, prec, first_axis, 1);
 }
 
+int launch_fft_axis_batch(psx_ctx* ctx, void* grid, int N, int sign, int prec, int axis, int nbox) {
+  const FftPlan* p;
+  PSX_TRY(get_fft_plan(ctx, N, &p));
+  if (prec == PSX_PREC_F64) {
+    cplx<double>* g = (cplx<double>*)grid;
+    return sign > 0 ? run_pass<double, 1>(ctx, p, g, g, axis, nbox)
+                    : run_pass<double, -1>(ctx, p, g, g, axis, nbox);
+  }
+  cplx<float>* g = (cplx<float>*)grid;
+  return sign > 0 ? run_pass<float, 1>(ctx, p, g, g, axis, nbox)
+                  : run_pass<float, -1>(ctx, p, g, g, axis, nbox);
+}
+
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec) {
   return launch_fft3d_axes(ctx, grid, N, sign, prec, 0);
 }

@@ -525,6 +525,48 @@ template <typename T> struct FcombCoef {
   T cf[PSX_MAX_BATCH];
 };
 
+// Iteration (ix, iy, iz) of estimator.f:646-665: from the eight values v[s]
+// at (xs[s>>2], ys[(s>>1)&1], zs[s&1]) (xs = {ix, -ix mod N}, ...) the
+// eight stores in the reference's order; m[canon(xb, yb, zb)] is then the
+// value the iteration leaves at (xs[xb], ys[yb], zs[zb]) -- where indices
+// alias (0 and N/2) the last store wins.
+struct FcombSlots {
+  bool ax, ay, az;
+  __device__ __forceinline__ int canon(int xb, int yb, int zb) const {
+    return ((xb && !ax) ? 4 : 0) + ((yb && !ay) ? 2 : 0) + ((zb && !az) ? 1 : 0);
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ void fcomb_iteration(cplx<T> (&m)[8], const FcombSlots& sl, int ix,
+                                                int iy, int iz, T cf, const c64* __restrict__ rec,
+                                                const float* __restrict__ Wf,
+                                                const double* __restrict__ Wd) {
+  // twiddles in double (estimator.f:650-653), rounded to T
+  const c64 xr = rec[ix], yr = rec[iy], zr = rec[iz];
+  const c64 ixr = mk(-xr.im, xr.re);
+  const c64 cma = ixr * yr * zr;
+  const c64 cmb = ixr * yr * conj(zr);
+  const c64 cmc = ixr * conj(yr) * zr;
+  const c64 cmd = ixr * conj(yr * zr);
+  const cplx<T> one = mk(T(1), T(0));
+  auto cv = [](c64 a) { return mk((T)a.re, (T)a.im); };
+  const cplx<T> A = cv(cma), B = cv(cmb), C = cv(cmc), D = cv(cmd);
+  const T cfac = cf / (window_val<T>(Wf, Wd, ix) * window_val<T>(Wf, Wd, iy) * window_val<T>(Wf, Wd, iz));
+  const cplx<T> c000 = m[0] * (one - A) + conj(m[7]) * (one + A);
+  const cplx<T> c001 = m[1] * (one - B) + conj(m[6]) * (one + B);
+  const cplx<T> c010 = m[2] * (one - C) + conj(m[5]) * (one + C);
+  const cplx<T> c011 = m[3] * (one - D) + conj(m[4]) * (one + D);
+  slot_put(m, sl.canon(0, 0, 0), c000 * cfac);
+  slot_put(m, sl.canon(0, 0, 1), c001 * cfac);
+  slot_put(m, sl.canon(0, 1, 0), c010 * cfac);
+  slot_put(m, sl.canon(0, 1, 1), c011 * cfac);
+  slot_put(m, sl.canon(1, 0, 0), conj(slot_get(m, sl.canon(0, 1, 1))));
+  slot_put(m, sl.canon(1, 0, 1), conj(slot_get(m, sl.canon(0, 1, 0))));
+  slot_put(m, sl.canon(1, 1, 0), conj(slot_get(m, sl.canon(0, 0, 1))));
+  slot_put(m, sl.canon(1, 1, 1), conj(slot_get(m, sl.canon(0, 0, 0))));
+}
+
 // Box blockIdx.y of a batch: grid F [box][N^3], half field Hh [box][N^2 (N/2+1)].
 template <typename T>
 __global__ __launch_bounds__(256) void k_fcomb(const cplx<T>* __restrict__ F, cplx<T>* __restrict__ Hh, int N,
@@ -543,52 +585,160 @@ __global__ __launch_bounds__(256) void k_fcomb(const cplx<T>* __restrict__ F, cp
   const int xs[2] = {ix, (N - ix) % N};
   const int ys[2] = {iy, (N - iy) % N};
   const int zs[2] = {iz, (N - iz) % N};
-  const bool ax = xs[0] == xs[1], ay = ys[0] == ys[1], az = zs[0] == zs[1];
-  // canonical slot of (xb, yb, zb)
-  auto canon = [&](int xb, int yb, int zb) {
-    return ((xb && !ax) ? 4 : 0) + ((yb && !ay) ? 2 : 0) + ((zb && !az) ? 1 : 0);
-  };
-  cplx<T> v[8];
+  const FcombSlots sl{xs[0] == xs[1], ys[0] == ys[1], zs[0] == zs[1]};
+  cplx<T> m[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     const int xb = s >> 2, yb = (s >> 1) & 1, zb = s & 1;
-    v[s] = F[((int64_t)zs[zb] * N + ys[yb]) * N + xs[xb]];
+    m[s] = F[((int64_t)zs[zb] * N + ys[yb]) * N + xs[xb]];
   }
-  // twiddles in double (estimator.f:650-653), rounded to T
-  const c64 xr = rec[ix], yr = rec[iy], zr = rec[iz];
-  const c64 ixr = mk(-xr.im, xr.re);
-  const c64 cma = ixr * yr * zr;
-  const c64 cmb = ixr * yr * conj(zr);
-  const c64 cmc = ixr * conj(yr) * zr;
-  const c64 cmd = ixr * conj(yr * zr);
-  const cplx<T> one = mk(T(1), T(0));
-  auto cv = [](c64 a) { return mk((T)a.re, (T)a.im); };
-  const cplx<T> A = cv(cma), B = cv(cmb), C = cv(cmc), D = cv(cmd);
-  const T cfac = cf / (window_val<T>(Wf, Wd, ix) * window_val<T>(Wf, Wd, iy) * window_val<T>(Wf, Wd, iz));
-  const cplx<T> c000 = v[0] * (one - A) + conj(v[7]) * (one + A);
-  const cplx<T> c001 = v[1] * (one - B) + conj(v[6]) * (one + B);
-  const cplx<T> c010 = v[2] * (one - C) + conj(v[5]) * (one + C);
-  const cplx<T> c011 = v[3] * (one - D) + conj(v[4]) * (one + D);
-  cplx<T> m[8];
-#pragma unroll
-  for (int s = 0; s < 8; ++s) m[s] = v[s];
-  slot_put(m, canon(0, 0, 0), c000 * cfac);
-  slot_put(m, canon(0, 0, 1), c001 * cfac);
-  slot_put(m, canon(0, 1, 0), c010 * cfac);
-  slot_put(m, canon(0, 1, 1), c011 * cfac);
-  slot_put(m, canon(1, 0, 0), conj(slot_get(m, canon(0, 1, 1))));
-  slot_put(m, canon(1, 0, 1), conj(slot_get(m, canon(0, 1, 0))));
-  slot_put(m, canon(1, 1, 0), conj(slot_get(m, canon(0, 0, 1))));
-  slot_put(m, canon(1, 1, 1), conj(slot_get(m, canon(0, 0, 0))));
+  fcomb_iteration(m, sl, ix, iy, iz, cf, rec, Wf, Wd);
   // half-field elements owned by this iteration: x = ix, y in {iy,-iy}, z in {iz,-iz}
 #pragma unroll
   for (int yb = 0; yb < 2; ++yb) {
 #pragma unroll
     for (int zb = 0; zb < 2; ++zb) {
-      if ((yb && ay) || (zb && az)) continue;
-      Hh[((int64_t)zs[zb] * N + ys[yb]) * Nh + ix] = slot_get(m, canon(0, yb, zb));
+      if ((yb && sl.ay) || (zb && sl.az)) continue;
+      Hh[((int64_t)zs[zb] * N + ys[yb]) * Nh + ix] = slot_get(m, sl.canon(0, yb, zb));
     }
   }
+}
+
+// The z pass of the backward grid FFT with fcomb as its epilogue.  The
+// half-field line (x, y), x <= N/2, is the fcomb of the lines (x, y) and
+// (-x, -y) after their z transforms: every value the store order of
+// iteration (x, |y|, |z|) reads or leaves at (x, y, z) lies on one of them
+// (where x, y or z alias, the other lines of the iteration coincide with
+// these two).  A tile holds tp such line pairs, consecutive x at one y; the
+// line transforms are those of k_fft_lines, the combination that of k_fcomb.
+template <typename T, int NC>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_zcomb(const cplx<T>* __restrict__ F,
+                                                            cplx<T>* __restrict__ Hh, FftShape sh,
+                                                            const cplx<T>* __restrict__ tw,
+                                                            FcombCoef<T> coef,
+                                                            const c64* __restrict__ rec,
+                                                            const float* __restrict__ Wf,
+                                                            const double* __restrict__ Wd) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int N = n_of<NC>(sh), Nh = N / 2 + 1, lstride = N + 1;
+  const int64_t NN = (int64_t)N * N;
+  F += (int64_t)blockIdx.y * NN * N;
+  Hh += (int64_t)blockIdx.y * NN * Nh;
+  const T cf = coef.cf[blockIdx.y];
+  constexpr int ML = Tile<T, NC>::MAXLINES;
+  const int tl = NC > 0 ? ML : tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES);
+  const int tp = tl / 2;  // line pairs per tile
+  const int ntx = (Nh + tp - 1) / tp;
+  const int y = (int)(blockIdx.x / ntx);
+  const int x0 = (int)(blockIdx.x - (int64_t)y * ntx) * tp;
+  const int np = min(tp, Nh - x0);
+  const int yb_line = (N - y) % N;
+  // line 2c: (x0 + c, y); line 2c + 1: (-(x0 + c), -y); element n along z
+  batched_fill<8, cplx<T>>(
+      2 * tp * N,
+      [&](int idx) {
+        const int n = idx / (2 * tp), l = idx - n * (2 * tp);
+        const int c = l >> 1;
+        if (c >= np) return mk(T(0), T(0));
+        const int x = (l & 1) ? (N - (x0 + c)) % N : x0 + c;
+        const int yy = (l & 1) ? yb_line : y;
+        return F[((int64_t)n * N + yy) * N + x];
+      },
+      [&](int idx, cplx<T> v) {
+        const int n = idx / (2 * tp), l = idx - n * (2 * tp);
+        if ((l >> 1) < np) lds[l * lstride + n] = v;
+      });
+  cplx<T>* twl = lds + tl * lstride;
+  stage_twiddles(twl, tw, N);
+  __syncthreads();
+  fft_tile<NC, 1, T, ML, TileCfg<T>::ELEMS>(lds, sh, 2 * np, lstride, twl);
+  const int yb = y > N / 2 ? 1 : 0, iy = yb ? N - y : y;
+  for (int idx = threadIdx.x; idx < np * N; idx += FFT_THREADS) {
+    const int z = idx / np, c = idx - z * np;
+    const int ix = x0 + c;
+    const int zb = z > N / 2 ? 1 : 0, iz = zb ? N - z : z;
+    const int xs[2] = {ix, (N - ix) % N};
+    const int ys[2] = {iy, (N - iy) % N};
+    const int zs[2] = {iz, (N - iz) % N};
+    const FcombSlots sl{xs[0] == xs[1], ys[0] == ys[1], zs[0] == zs[1]};
+    const cplx<T>* la = lds + (2 * c) * lstride;      // (ix, y)
+    const cplx<T>* lb = lds + (2 * c + 1) * lstride;  // (-ix, -y)
+    cplx<T> out;
+    if (!(sl.ax || sl.ay || sl.az)) {
+      // no index aliases: the element keeps its own combination c_{0,yb,zb}
+      // = v (1 - t) + conj(v') (1 + t) times cfac, v = F(k), v' = F(-k), with
+      // the twiddle t of k_fcomb's cma/cmb/cmc/cmd (the same operations)
+      const c64 xr = rec[ix], yr = rec[iy], zr = rec[iz];
+      const c64 ixr = mk(-xr.im, xr.re);
+      const c64 yv = yb ? conj(yr) : yr;
+      c64 t64;
+      if (yb && zb) t64 = ixr * conj(yr * zr);
+      else t64 = ixr * yv * (zb ? conj(zr) : zr);
+      const cplx<T> t = mk((T)t64.re, (T)t64.im);
+      const cplx<T> one = mk(T(1), T(0));
+      const T cfac = cf / (window_val<T>(Wf, Wd, ix) * window_val<T>(Wf, Wd, iy) *
+                           window_val<T>(Wf, Wd, iz));
+      out = (la[z] * (one - t) + conj(lb[(N - z) % N]) * (one + t)) * cfac;
+    } else {
+      cplx<T> m[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int X = xs[s >> 2], Y = ys[(s >> 1) & 1], Z = zs[s & 1];
+        // every (X, Y) the iteration needs here is one of the two lines
+        m[s] = (X == ix && Y == y) ? la[Z] : lb[Z];
+      }
+      fcomb_iteration(m, sl, ix, iy, iz, cf, rec, Wf, Wd);
+      out = slot_get(m, sl.canon(0, yb, zb));
+    }
+    Hh[((int64_t)z * N + y) * Nh + ix] = out;
+  }
+}
+
+template <typename T, int NC>
+static int zcomb_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* F, cplx<T>* Hh, int nbox,
+                   const FcombCoef<T>& c) {
+  const int N = p->shape.N, Nh = N / 2 + 1;
+  const int tl = tile_lines_host<T>(N), tp = tl / 2;
+  const size_t lds = ((size_t)tl * (N + 1) + N) * sizeof(cplx<T>);
+  auto kern = k_fft_zcomb<T, NC>;
+  PSX_TRY(set_lds(kern, lds));
+  const dim3 grid((unsigned)((int64_t)N * ((Nh + tp - 1) / tp)), (unsigned)nbox);
+  TimedLaunch timed(ctx, "k_fft_zcomb");
+  hipLaunchKernelGGL(kern, grid, dim3(FFT_THREADS), lds, ctx->stream, F, Hh, p->shape,
+                     plan_tw<T>(p), c, (const c64*)p->rec, p->Wf, p->Wd);
+  PSX_CHECK_LAUNCH();
+  return PSX_OK;
+}
+
+#define PSX_CASE_N(n) \
+  case n: return zcomb_n<T, n>(ctx, p, F, Hh, nbox, c);
+
+template <typename T>
+static int zcomb_t(psx_ctx* ctx, const FftPlan* p, const cplx<T>* F, cplx<T>* Hh, int nbox,
+                   const FcombCoef<T>& c) {
+  switch (p->shape.N) {
+    PSX_FFT_SIZES(PSX_CASE_N)
+    default: return zcomb_n<T, 0>(ctx, p, F, Hh, nbox, c);
+  }
+}
+#undef PSX_CASE_N
+
+int launch_fft_zcomb_batch(psx_ctx* ctx, const void* F, void* Hh, int N, int nbox,
+                           const double* sumw, int prec) {
+  PSX_REQUIRE(nbox >= 1 && nbox <= PSX_MAX_BATCH, "batch of 1 to 16 boxes");
+  PSX_REQUIRE(tile_lines_host<double>(N) >= 2, "grid too large for the fused z pass");
+  const FftPlan* p;
+  PSX_TRY(get_fft_plan(ctx, N, &p));
+  if (prec == PSX_PREC_F64) {
+    FcombCoef<double> c{};
+    // cf = 1/(6^3 * 4 * N) with N = sum w as float32 (estimator.f:615)
+    for (int b = 0; b < nbox; ++b) c.cf[b] = 1.0 / (216.0 * 4.0 * (double)(float)sumw[b]);
+    return zcomb_t<double>(ctx, p, (const cplx<double>*)F, (cplx<double>*)Hh, nbox, c);
+  }
+  FcombCoef<float> c{};
+  for (int b = 0; b < nbox; ++b) c.cf[b] = 1.0f / (864.0f * (float)sumw[b]);
+  return zcomb_t<float>(ctx, p, (const cplx<float>*)F, (cplx<float>*)Hh, nbox, c);
 }
 
 // nbox grids back to back -> nbox half fields back to back.

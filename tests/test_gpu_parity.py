@@ -151,8 +151,10 @@ def test_fcomb_matches_oracle(N, prec):
 
 # ------------------------------------------------------- FFT_periodic ------
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
-@pytest.mark.parametrize("N,weights", [(32, False), (48, True)])
+@pytest.mark.parametrize("N,weights", [(32, False), (48, True), (30, True), (40, False)])
 def test_fft_periodic_matches_oracle(prec, N, weights):
+    # 32, 48: compile-time FFT sizes; 30, 40: the runtime-N engine (the z
+    # pass fused with fcomb in both)
     _say("FFT_periodic", prec, N, weights)
     Lbox = 500.0
     xyz, w = _particles(5000, Lbox, 3, weights)
@@ -160,6 +162,29 @@ def test_fft_periodic_matches_oracle(prec, N, weights):
     out = P.FFT_periodic(xyz, w=w, Lbox=Lbox, Ngrid=N, precision=prec)
     assert out.shape == (N // 2 + 1, N, N)
     assert _rel(out, ref) < (1e-10 if prec == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("N", [24, 64, 360])
+def test_fused_zcomb_equals_separate_passes(prec, N):
+    """FFT_periodic's z pass with fcomb fused equals the separate stages
+    (assignment, three line passes, k_fcomb) on the same particles: the
+    line transforms and the combination are the same arithmetic."""
+    _say("fused z pass + fcomb", prec, N)
+    Lbox = 700.0
+    xyz, w = _particles(30000, Lbox, 8, weights=True)
+    fused = P.FFT_periodic(xyz, w=w, Lbox=Lbox, Ngrid=N, precision=prec)
+    x = torch.from_numpy(np.ascontiguousarray(xyz.astype(np.float32))).to(DEV)
+    wd = torch.from_numpy(w.astype(np.float32)).to(DEV)
+    g = torch.empty((N, N, N), dtype=_cdt(prec), device=DEV)
+    L.call("psx_assign_quad", _h(), L.ptr(x), L.ptr(wd), x.shape[1], N,
+           float(np.float32(N / Lbox)), 0.0, 0.0, float(np.float32(Lbox * (1 - 1e-6))),
+           float(np.abs(w).sum()), L.ptr(g), L.PREC[prec])
+    L.call("psx_fft3d", _h(), L.ptr(g), N, 1, L.PREC[prec])
+    half = torch.empty((N, N, N // 2 + 1), dtype=_cdt(prec), device=DEV)
+    L.call("psx_fcomb_periodic", _h(), L.ptr(g), L.ptr(half), N, float(np.sum(w)), L.PREC[prec])
+    sep = half.cpu().numpy().transpose(2, 1, 0)
+    assert _rel(fused, sep) < (1e-13 if prec == "fp64" else 1e-6)
 
 
 def test_reflect_matches_oracle():
