@@ -38,17 +38,19 @@
 
 namespace psx {
 
-// Waves per workgroup and pair tiles per wave: 8 x 3 (two waves per SIMD,
-// 256 VGPRs each, so the float32 and float64 accumulators of three tiles stay
-// in registers through the FFT phase).
+// Waves per workgroup and pair tiles per wave: 12 x 2 (three waves per SIMD,
+// at most 168 VGPRs each; the float32 and float64 accumulators of two tiles
+// stay in registers through the FFT phase).  A SIMD overlaps little of one
+// wave's VALU work with another's MFMAs, so its busy time is close to the sum
+// of both; the third wave fills the LDS and barrier waits.
 #ifndef PSX_XC_WAVES
-#define PSX_XC_WAVES 8
+#define PSX_XC_WAVES 12
 #endif
 #ifndef PSX_XC_MAXPT
-#define PSX_XC_MAXPT 3
+#define PSX_XC_MAXPT 2
 #endif
 constexpr int XC_WAVES = PSX_XC_WAVES, XC_THREADS = XC_WAVES * 64, XC_MAXPT = PSX_XC_MAXPT;
-constexpr int XC_NPF = 3;  // gathered (line, kx) items per thread and row
+constexpr int XC_NPF = (1536 + XC_THREADS - 1) / XC_THREADS;  // gathered (line, kx) items per thread and row
 int xc_waves_per_group() { return XC_WAVES; }
 int xc_maxpt() { return XC_MAXPT; }
 #ifndef PSX_XC_FLUSH
@@ -83,7 +85,7 @@ static size_t xc_lds_bytes(int ns, int nlines_ct) {
 template <int N, int NLINES, int STAGE, int NS, int TW, typename Emit>
 __device__ __forceinline__ void xc_stages(cplx<float>* lines, int nlines,
                                           const cplx<float>* __restrict__ tw,
-                                          const int* __restrict__ kxl_line, Emit& emit) {
+                                          const int* __restrict__ kxl_line, Emit& emit, int tid) {
   constexpr FftShape sh = fft_shape_of(N);
   if constexpr (STAGE < sh.nstages) {
     constexpr int R = sh.radix[STAGE];
@@ -96,7 +98,7 @@ __device__ __forceinline__ void xc_stages(cplx<float>* lines, int nlines,
     int kk[MAXB], jj[MAXB], ll[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-      const int b = (int)threadIdx.x + q * XC_THREADS;
+      const int b = tid + q * XC_THREADS;
       const int line = b / NB, j = b - line * NB;
       ll[q] = line;
       jj[q] = j;
@@ -125,7 +127,7 @@ __device__ __forceinline__ void xc_stages(cplx<float>* lines, int nlines,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-      const int b = (int)threadIdx.x + q * XC_THREADS;
+      const int b = tid + q * XC_THREADS;
       if (b < nbf) {
         const int k = kk[q];
         cplx<float>* base = lines + ll[q] * L;
@@ -138,8 +140,26 @@ __device__ __forceinline__ void xc_stages(cplx<float>* lines, int nlines,
       }
     }
     __syncthreads();
-    xc_stages<N, NLINES, STAGE + 1, NS * R, TW + NS * (R - 1)>(lines, nlines, tw, kxl_line, emit);
+    xc_stages<N, NLINES, STAGE + 1, NS * R, TW + NS * (R - 1)>(lines, nlines, tw, kxl_line, emit,
+                                                               tid);
   }
+}
+
+// The MFMA blocks of one pair tile at one k step: for each column tile t in
+// the mask m, acc_t += Ah Bh_t + Ah Bl_t + Al Bh_t (from zero when first),
+// each accumulator updated in place under its own wave-uniform branch.
+template <int NLT>
+__device__ __forceinline__ void xc_mfma_blocks(f4v (&acc)[NLT], const bf8v& ah, const bf8v& al,
+                                               const bf8v (&bh)[NLT], const bf8v (&bl)[NLT],
+                                               bool first, int m) {
+  const f4v zero = {0, 0, 0, 0};
+#pragma unroll
+  for (int t = 0; t < NLT; ++t)
+    if ((m >> t) & 1) {
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], first ? zero : acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[t], 0, 0, 0);
+    }
 }
 
 // T2: per shell s, rows (z*N + y) of kxw_s complex values at offset off[s];
@@ -279,16 +299,16 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
   // The float rows 0 .. 2 nlines - 1 split into the bf16 hi/lo planes, four
   // consecutive points per thread; positions N..KC-1 of the even rows lie
   // inside the FFT lines and are zeroed here first.
-  auto split_rows = [&]() {
+  auto split_rows = [&](int tid) {
     if constexpr (KC > N) {
-      for (int i = threadIdx.x; i < nlines * (KC - N); i += XC_THREADS) {
+      for (int i = tid; i < nlines * (KC - N); i += XC_THREADS) {
         const int l = i / (KC - N);
         rows[2 * l * L + N + (i - l * (KC - N))] = 0.0f;
       }
       __syncthreads();
     }
     constexpr int Q = KC / 4;
-    for (int i = threadIdx.x; i < 2 * nlines * Q; i += XC_THREADS) {
+    for (int i = tid; i < 2 * nlines * Q; i += XC_THREADS) {
       const int s = i / Q, k = (i - s * Q) * 4;
       const f4v x = *reinterpret_cast<const f4v*>(rows + s * L + k);
       u2v h, l;
@@ -321,14 +341,28 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
     __syncthreads();
     if (row + 1 < r_end) fetch(row + 1);
     PSX_XC_TICK(0);
-    xc_stages<N, NLINES, 0, 1, 0>(lines, nlines, twl, kxl_l, emit);
+    // the thread index passes through a register each row, so the compiler
+    // recomputes the FFT and split indices per row instead of holding them
+    // in registers through the contraction
+    const int tid = (int)in_register(threadIdx.x);
+    xc_stages<N, NLINES, 0, 1, 0>(lines, nlines, twl, kxl_l, emit, tid);
     PSX_XC_TICK(1);
-    split_rows();
+    split_rows(tid);
     __syncthreads();
     PSX_XC_TICK(2);
-    const bool fresh = chain == 0;
+    // the k steps of the row: the B fragments of the step, then per pair tile
+    // (a slot with an empty mask is skipped) its products, split, and MFMAs
+    if constexpr (PSX_XC_FLUSH > 1) {
+      if (chain == 0) {
 #pragma unroll
-    for (int k0 = 0; k0 < KC; k0 += 32) {
+        for (int q = 0; q < XC_MAXPT; ++q)
+#pragma unroll
+          for (int t = 0; t < NLT; ++t) acc[q][t] = zero;
+      }
+    }
+#pragma unroll
+    for (int kb = 0; kb < KC / 32; ++kb) {
+      const int k0 = kb * 32;
       bf8v bh[NLT], bl[NLT];
 #pragma unroll
       for (int t = 0; t < NLT; ++t) {
@@ -337,20 +371,12 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
       }
 #pragma unroll
       for (int q = 0; q < XC_MAXPT; ++q) {
-        if (q >= nq) break;
+        if (msk[q] == 0) continue;
         const f4v* a4 = reinterpret_cast<const f4v*>(rows + oi[q] + k0);
         const f4v* b4 = reinterpret_cast<const f4v*>(rows + oj[q] + k0);
         bf8v ah, al;
         product_split_bf16(a4[0], a4[4], b4[0], b4[4], neg_lo, neg_hi, ah, al);
-#pragma unroll
-        for (int t = 0; t < NLT; ++t) {
-          if ((msk[q] >> t) & 1) {
-            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                ah, bh[t], (fresh && k0 == 0) ? zero : acc[q][t], 0, 0, 0);
-            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[q][t], 0, 0, 0);
-            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[q][t], 0, 0, 0);
-          }
-        }
+        xc_mfma_blocks<NLT>(acc[q], ah, al, bh, bl, PSX_XC_FLUSH == 1 && kb == 0, msk[q]);
       }
     }
     PSX_XC_TICK(3);

@@ -37,7 +37,7 @@ CANDIDATES = {
     # split-BF16 contraction fused with the shell x pass (psx_xcontract.hip)
     "xc_flush2": {"PSX_XC_FLUSH": 2},
     "xc_flush4": {"PSX_XC_FLUSH": 4},
-    "xc_w12x2": {"PSX_XC_WAVES": 12, "PSX_XC_MAXPT": 2},
+    "xc_w8x3": {"PSX_XC_WAVES": 8, "PSX_XC_MAXPT": 3},
     # FFT tiles and radix order (front end, shell z/y passes, fused x pass)
     "fft_lines8": {"PSX_TILE_LINES_F32": 8},
     "fft_order_9_5_8": {"PSX_FFT_ORDER": 2},
