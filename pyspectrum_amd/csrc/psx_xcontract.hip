@@ -76,14 +76,15 @@ static size_t xc_lds_bytes(int ns, int nlines_ct) {
   return b;
 }
 
-// One radix stage of the line FFTs (forward).  Butterflies are numbered
-// across all lines (b = line * N/R + j) and dealt to the threads in order, so
-// the threads of a partly filled last pass sit in as few waves as possible and
-// the other waves skip it.  FIRST: the input entries kxl <= n <= N - kxl of a
-// line are zero (never stored); LAST: the outputs go to emit(line, position,
-// value) instead of back to the line.
+// One radix stage of the line FFTs (forward), out of place: the stage reads
+// the lines at `in` and writes them at `out` (LAST: to emit(line, position,
+// value)), so one barrier per stage suffices; the next stage swaps the two.
+// Butterflies are numbered across all lines (b = line * N/R + j) and dealt to
+// the threads in order, so the threads of a partly filled last pass sit in as
+// few waves as possible and the other waves skip it.  FIRST: the input
+// entries kxl <= n <= N - kxl of a line are zero (never stored).
 template <int N, int NLINES, int STAGE, int NS, int TW, typename Emit>
-__device__ __forceinline__ void xc_stages(cplx<float>* lines, int nlines,
+__device__ __forceinline__ void xc_stages(const cplx<float>* in, cplx<float>* out, int nlines,
                                           const cplx<float>* __restrict__ tw,
                                           const int* __restrict__ kxl_line, Emit& emit, int tid) {
   constexpr FftShape sh = fft_shape_of(N);
@@ -94,54 +95,42 @@ __device__ __forceinline__ void xc_stages(cplx<float>* lines, int nlines,
     constexpr bool FIRST = STAGE == 0, LAST = STAGE == sh.nstages - 1;
     constexpr int L = XcGeom<N>::L;
     const int nbf = nlines * NB;
-    cplx<float> v[MAXB][R];
-    int kk[MAXB], jj[MAXB], ll[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
       const int b = tid + q * XC_THREADS;
-      const int line = b / NB, j = b - line * NB;
-      ll[q] = line;
-      jj[q] = j;
-      kk[q] = 0;
       if (b < nbf) {
-        const cplx<float>* base = lines + line * L;
+        const int line = b / NB, j = b - line * NB;
+        const cplx<float>* base = in + line * L;
         const int kxl = FIRST ? kxl_line[line] : 0;
+        cplx<float> v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int n = j + r * NB;
           if constexpr (FIRST)
-            v[q][r] = (n < kxl || n > N - kxl) ? base[n] : mk(0.0f, 0.0f);
+            v[r] = (n < kxl || n > N - kxl) ? base[n] : mk(0.0f, 0.0f);
           else
-            v[q][r] = base[n];
+            v[r] = base[n];
         }
+        int k = 0;
         if constexpr (NS > 1) {
-          const int k = j % NS;
-          kk[q] = k;
+          k = j % NS;
           const cplx<float>* w = tw + TW + k * (R - 1);
 #pragma unroll
-          for (int r = 1; r < R; ++r) v[q][r] = v[q][r] * w[r - 1];
+          for (int r = 1; r < R; ++r) v[r] = v[r] * w[r - 1];
         }
-        dftR<R, -1>(v[q]);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < MAXB; ++q) {
-      const int b = tid + q * XC_THREADS;
-      if (b < nbf) {
-        const int k = kk[q];
-        cplx<float>* base = lines + ll[q] * L;
+        dftR<R, -1>(v);
+        cplx<float>* ob = out + line * L;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int pos = (jj[q] - k) * R + k + r * NS;
-          if constexpr (LAST) emit(ll[q], pos, v[q][r]);
-          else base[pos] = v[q][r];
+          const int pos = (j - k) * R + k + r * NS;
+          if constexpr (LAST) emit(line, pos, v[r]);
+          else ob[pos] = v[r];
         }
       }
     }
     __syncthreads();
-    xc_stages<N, NLINES, STAGE + 1, NS * R, TW + NS * (R - 1)>(lines, nlines, tw, kxl_line, emit,
-                                                               tid);
+    xc_stages<N, NLINES, STAGE + 1, NS * R, TW + NS * (R - 1)>(out, const_cast<cplx<float>*>(in),
+                                                               nlines, tw, kxl_line, emit, tid);
   }
 }
 
@@ -184,7 +173,7 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
   const int nlines = (ns + 1) / 2;
   const int nrows = 2 * (ns / 2 + 1);
   float* rows = reinterpret_cast<float*>(smem_raw);
-  cplx<float>* lines = reinterpret_cast<cplx<float>*>(smem_raw);
+  cplx<float>* const lines_a = reinterpret_cast<cplx<float>*>(smem_raw);
   unsigned short* ldh = reinterpret_cast<unsigned short*>(rows + (size_t)nrows * L);
   unsigned short* ldl = ldh + (size_t)(ns + 1) * LDWH;
   cplx<float>* twl = reinterpret_cast<cplx<float>*>(ldl + (size_t)(ns + 1) * LDWH);
@@ -193,9 +182,18 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
   int* s_kxl = s_kxw + 64;
   int* s_pad = s_kxl + 64;
   int64_t* s_off = reinterpret_cast<int64_t*>(s_pad + 64);
+  // The out-of-place x pass alternates between the lines over the float rows
+  // (A) and lines over the bf16 planes (B), both free while a row is
+  // transformed; the input goes where the last stage reads from B, whose
+  // output (the float rows) lands in A.
+  cplx<float>* const lines_b = reinterpret_cast<cplx<float>*>(ldh);
+  constexpr bool IN_B = fft_shape_of(N).nstages % 2 == 1;
+  cplx<float>* const lines = IN_B ? lines_b : lines_a;
+  cplx<float>* const lines_o = IN_B ? lines_a : lines_b;
 
-  // zero rows and planes once: the zero row (index ns), the padding columns
-  // N..KC-1 of the planes and of the odd rows are never written again
+  // zero rows and planes once: the zero row (index ns) and the padding
+  // columns N..KC-1 of the odd rows are never written again (the planes,
+  // zero row and padding included, are rewritten from the rows every row)
   for (int i = threadIdx.x; i < nrows * L; i += XC_THREADS) rows[i] = 0.0f;
   for (int i = threadIdx.x; i < 2 * (ns + 1) * LDWH; i += XC_THREADS) ldh[i] = 0;
   for (int i = threadIdx.x; i < N; i += XC_THREADS) twl[i] = tw[i];
@@ -296,9 +294,9 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
       rows[s1 * L + pos] = s1 < ns ? v.im : 0.0f;
     }
   } emit{rows, ns};
-  // The float rows 0 .. 2 nlines - 1 split into the bf16 hi/lo planes, four
-  // consecutive points per thread; positions N..KC-1 of the even rows lie
-  // inside the FFT lines and are zeroed here first.
+  // The float rows 0 .. ns (row ns is zero) split into the bf16 hi/lo planes,
+  // four consecutive points per thread; positions N..KC-1 of the even rows
+  // lie inside the FFT lines and are zeroed here first.
   auto split_rows = [&](int tid) {
     if constexpr (KC > N) {
       for (int i = tid; i < nlines * (KC - N); i += XC_THREADS) {
@@ -308,7 +306,7 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
       __syncthreads();
     }
     constexpr int Q = KC / 4;
-    for (int i = tid; i < 2 * nlines * Q; i += XC_THREADS) {
+    for (int i = tid; i < (ns + 1) * Q; i += XC_THREADS) {
       const int s = i / Q, k = (i - s * Q) * 4;
       const f4v x = *reinterpret_cast<const f4v*>(rows + s * L + k);
       u2v h, l;
@@ -345,7 +343,7 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
     // recomputes the FFT and split indices per row instead of holding them
     // in registers through the contraction
     const int tid = (int)in_register(threadIdx.x);
-    xc_stages<N, NLINES, 0, 1, 0>(lines, nlines, twl, kxl_l, emit, tid);
+    xc_stages<N, NLINES, 0, 1, 0>(lines, lines_o, nlines, twl, kxl_l, emit, tid);
     PSX_XC_TICK(1);
     split_rows(tid);
     __syncthreads();
