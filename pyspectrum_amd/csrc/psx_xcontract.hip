@@ -254,18 +254,29 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
   }
   __syncthreads();
 
+  // per item, the T2 offsets of its two entries in row 0 and their row
+  // strides (stride -1: the entry is outside its shell's kx range, zero)
+  int64_t g_off[XC_NPF][2];
+  int g_w[XC_NPF][2];
+#pragma unroll
+  for (int u = 0; u < XC_NPF; ++u) {
+    const int p = it_p[u], kx = it_kx[u];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int sh = 2 * p + h, shc = max(0, min(sh, ns - 1));
+      const bool in = p >= 0 && sh < ns && kx < s_kxl[shc];
+      g_off[u][h] = in ? s_off[shc] + kx : 0;
+      g_w[u][h] = in ? s_kxw[shc] : -1;
+    }
+  }
   cplx<float> pa[XC_NPF], pb[XC_NPF];
   auto fetch = [&](int row) {
 #pragma unroll
     for (int u = 0; u < XC_NPF; ++u) {
       pa[u] = mk(0.0f, 0.0f);
       pb[u] = mk(0.0f, 0.0f);
-      const int p = it_p[u], kx = it_kx[u];
-      if (p >= 0) {
-        const int s0 = 2 * p, s1 = 2 * p + 1;
-        if (kx < s_kxl[s0]) pa[u] = T2[s_off[s0] + (int64_t)row * s_kxw[s0] + kx];
-        if (s1 < ns && kx < s_kxl[s1]) pb[u] = T2[s_off[s1] + (int64_t)row * s_kxw[s1] + kx];
-      }
+      if (g_w[u][0] >= 0) pa[u] = T2[g_off[u][0] + (int64_t)row * g_w[u][0]];
+      if (g_w[u][1] >= 0) pb[u] = T2[g_off[u][1] + (int64_t)row * g_w[u][1]];
     }
   };
   // the two Hermitian rows a (shell 2p) and b (shell 2p+1) as one line
