@@ -251,9 +251,12 @@ def test_bk_raw_matches_oracle(prec, N, step, ncut, nmax):
     assert _rel(p0.cpu().numpy()[j0 - 1:], p0k[j0 - 1:]) < (1e-12 if prec == "fp64" else 1e-6)
 
 
-def test_bk_raw_error_scale_past_nyquist():
+@pytest.mark.parametrize("N,step,ncut,nmax", [(128, 3, 3, 40), (96, 2, 2, 46)])
+def test_bk_raw_error_scale_past_nyquist(N, step, ncut, nmax):
     """Ngrid 128, step 3, Ncut 3, Nmax 40 (the small grids of the batch
-    workload): shells 22-40 lie past the Nyquist sphere and hold only modes
+    workload; the fused kernel's compact-input layout) and Ngrid 96, 46
+    shells (its layout for more than 40 shells): shells 22-40 (24-46) lie
+    past the Nyquist sphere and hold only modes
     near the cube's corners, so their triangle sums cancel by orders of
     magnitude and a relative error says little.  What rounding can do to a
     sum is bounded by its absolute sum A_t = sum_x |I_i||I_j||I_l| (FP64
@@ -261,7 +264,6 @@ def test_bk_raw_error_scale_past_nyquist():
     round-off times A_t -- split-BF16 (16 significant bits per split operand,
     fused and staged kernels) and FP32 -- while a lost or wrong term would
     be of order A_t.  Triangles inside the sphere also meet the 1e-3 spec."""
-    N, step, ncut, nmax = 128, 3, 3, 40
     _say("bk_raw error scale past Nyquist", N, step, ncut, nmax)
     rng = np.random.default_rng(13)
     r = rng.standard_normal((N, N, N))
