@@ -38,9 +38,11 @@ struct FftShape {
   int radix[FFT_MAX_STAGES];
 };
 
-// Factor N = 2^a 3^b 5^c into stages: 5s, 9s, a 3, a 4 or 2, then 8s.
+// Factor N = 2^a 3^b 5^c into stages: 9s, a 3, 5s, a 4 or 2, then 8s
+// (order 2, the default: 9 x 5 x 8 at N = 360 measured fastest); order 0
+// puts the 5s first, order 1 the 8s.
 #ifndef PSX_FFT_ORDER
-#define PSX_FFT_ORDER 0
+#define PSX_FFT_ORDER 2
 #endif
 __host__ __device__ constexpr FftShape fft_shape_of(int N) {
   FftShape sh{N, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};

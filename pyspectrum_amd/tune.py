@@ -40,7 +40,7 @@ CANDIDATES = {
     "xc_w8x3": {"PSX_XC_WAVES": 8, "PSX_XC_MAXPT": 3},
     # FFT tiles and radix order (front end, shell z/y passes, fused x pass)
     "fft_lines8": {"PSX_TILE_LINES_F32": 8},
-    "fft_order_9_5_8": {"PSX_FFT_ORDER": 2},
+    "fft_order_5_9_8": {"PSX_FFT_ORDER": 0},
     # the staged contraction (grids or shell counts outside the fused kernel)
     "ct_w8x3": {"PSX_CT_WAVES_BF": 8, "PSX_CT_MAXPT_BF": 3},
     "shellx_p8": {"PSX_SHELL_X_PAIRS": 8},
