@@ -95,6 +95,16 @@ __device__ __forceinline__ float cr_sqrtf(float x) {
   return r;
 }
 
+// Workgroup index b (of nb) renumbered so that each XCD runs a contiguous
+// range: the dispatcher deals workgroups round-robin over the 8 XCDs, each
+// with its own L2, so tiles t and t + 1 that share cache lines would
+// otherwise be fetched by two L2s.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_local_block(int b, int nb) {
+  constexpr int NX = 8;
+  const int q = nb / NX, r = nb % NX, x = b % NX, i = b / NX;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // Shell index of |k|^2 = m: int(sqrt(m)/step + 0.5) in exact integers
 // (largest n with (2n-1)*step <= 2*sqrt(m)); step = 1 gives the P(k) bins.
 __host__ __device__ __forceinline__ int shell_of_m(int64_t m, int step) {

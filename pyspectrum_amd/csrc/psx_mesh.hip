@@ -630,8 +630,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_zcomb(const cplx<T>* __rest
   const int tl = NC > 0 ? ML : tile_lines_for(N, TileCfg<T>::ELEMS, TileCfg<T>::LINES);
   const int tp = tl / 2;  // line pairs per tile
   const int ntx = (Nh + tp - 1) / tp;
-  const int y = (int)(blockIdx.x / ntx);
-  const int x0 = (int)(blockIdx.x - (int64_t)y * ntx) * tp;
+  // neighbouring x tiles read the two halves of the same 128-byte lines
+  const int bx = xcd_local_block((int)blockIdx.x, (int)gridDim.x);
+  const int y = bx / ntx;
+  const int x0 = (bx - y * ntx) * tp;
   const int np = min(tp, Nh - x0);
   const int yb_line = (N - y) % N;
   // line 2c: (x0 + c, y); line 2c + 1: (-(x0 + c), -y); element n along z
