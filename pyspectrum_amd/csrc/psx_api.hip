@@ -454,6 +454,63 @@ int get_contract_plan(psx_ctx* ctx, int prec, int step, int ncut, int nmax,
     // blocks' worth of MFMA time) plus its output blocks
     load[best] += 3 + __builtin_popcount(ptmask[pt]);
   }
+  // Fused layout (one workgroup per CU): the waves of the workgroup are
+  // spread over the CU's four SIMDs (wave w on SIMD w % 4), and a SIMD issues
+  // its waves' VALU and MFMA work one at a time, so what bounds the group is
+  // the busiest SIMD.  Swap tiles (or a tile and an empty slot) between waves
+  // of different SIMDs while that lowers the group's (largest SIMD load, sum
+  // of squared SIMD loads).  Which wave runs a tile changes no result: its
+  // sums are formed in the same order.  (Measured at Ngrid 360, 40 shells:
+  // k_xcontract 6.13 -> 5.95 ms; the staged k_contract, several workgroups
+  // per CU, ran 4-6 % slower with it and keeps the per-wave balance.)
+  if (fused && WPG % 4 == 0) {
+    auto tcost = [&](int pt) { return pt < 0 ? 0 : 3 + __builtin_popcount(ptmask[pt]); };
+    for (int g = 0; g < ngroups; ++g) {
+      const int w0 = g * WPG;
+      auto score = [&](long long* sq) {
+        int sl[4] = {0, 0, 0, 0};
+        for (int w = w0; w < w0 + WPG; ++w) sl[(w - w0) % 4] += load[w];
+        *sq = 0;
+        int mx = 0;
+        for (int s = 0; s < 4; ++s) { mx = std::max(mx, sl[s]); *sq += (long long)sl[s] * sl[s]; }
+        return mx;
+      };
+      long long sq;
+      int mx = score(&sq);
+      for (bool improved = true; improved;) {
+        improved = false;
+        for (int a = w0 * maxpt; a < (w0 + WPG) * maxpt; ++a)
+          for (int b = a + 1; b < (w0 + WPG) * maxpt; ++b) {
+            const int wa = a / maxpt, wb = b / maxpt;
+            if ((wa - w0) % 4 == (wb - w0) % 4 || wave_pt[a] == wave_pt[b]) continue;
+            const int d = tcost(wave_pt[b]) - tcost(wave_pt[a]);
+            load[wa] += d; load[wb] -= d;
+            long long sq2;
+            const int mx2 = score(&sq2);
+            if (mx2 < mx || (mx2 == mx && sq2 < sq)) {
+              std::swap(wave_pt[a], wave_pt[b]);
+              std::swap(wave_mask[a], wave_mask[b]);
+              mx = mx2; sq = sq2; improved = true;
+            } else {
+              load[wa] -= d; load[wb] += d;
+            }
+          }
+      }
+    }
+    // slots in use first within each wave (the kernels count a wave's tiles)
+    for (int w = 0; w < nw; ++w)
+      for (int q = 0; q < maxpt; ++q)
+        for (int r = q + 1; r < maxpt && wave_pt[w * maxpt + q] < 0; ++r)
+          if (wave_pt[w * maxpt + r] >= 0) {
+            std::swap(wave_pt[w * maxpt + q], wave_pt[w * maxpt + r]);
+            std::swap(wave_mask[w * maxpt + q], wave_mask[w * maxpt + r]);
+          }
+    for (int w = 0; w < nw; ++w)
+      for (int q = 0; q < maxpt; ++q) {
+        const int pt = wave_pt[w * maxpt + q];
+        if (pt >= 0) { pt_wave[pt] = w; pt_q[pt] = q; }
+      }
+  }
   // output slots per group: the blocks that hold a triangle
   std::vector<int> wave_slot(nw * maxpt * 4, -1);
   std::vector<int> gslots(ngroups, 0);
