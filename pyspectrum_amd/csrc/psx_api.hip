@@ -206,8 +206,10 @@ int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, in
       for (int kx0 = 0; kx0 < xlim; kx0 += kt)
         if ((int64_t)kx0 * kx0 + (int64_t)kys * kys <= mm) zt.push_back({s, ky, kx0, 0});
     }
+    // the z pass wrote the rows kx0^2 + ky^2 <= mm of kx tile kx0: |ky| <= kyl
     for (int z = 0; z < N; ++z)
-      for (int kx0 = 0; kx0 < xlim; kx0 += kt) yt.push_back({s, z, kx0, 0});
+      for (int kx0 = 0; kx0 < xlim; kx0 += kt)
+        yt.push_back({s, z, kx0, (int)isqrt64(mm - (int64_t)kx0 * kx0)});
     for (int z = 0; z < N; ++z)
       for (int y0 = 0; y0 < N; y0 += 2 * xpairs) xt.push_back({s, z, y0, 0});
   }
@@ -235,7 +237,6 @@ int get_shell_plan(psx_ctx* ctx, int N, int step, int ncut, int nmax, int kt, in
   PSX_TRY(upload(ctx, xt, &sp->d_xtiles));
   PSX_TRY(upload(ctx, meta, &sp->d_meta));
   PSX_TRY(upload(ctx, sp->off, &sp->d_off));
-  PSX_TRY(upload(ctx, sp->mmax, &sp->d_mmax));
   *out = sp.get();
   ctx->shell_plans[key] = std::move(sp);
   return PSX_OK;

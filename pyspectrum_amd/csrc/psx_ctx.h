@@ -78,7 +78,7 @@ template <typename T> __host__ __device__ constexpr int shell_x_pairs(int NC) {
 
 // One pruned-pass tile: (shell slot, a, b) with meaning per pass.
 struct Tile3 {
-  int s, a, b, pad;
+  int s, a, b, c;  // c: y pass, the largest |ky| of the rows the z pass wrote
 };
 
 // Shell-field plan for (Ngrid, step, Ncut, Nmax).
@@ -96,7 +96,6 @@ struct ShellPlan {
   Tile3* d_xtiles = nullptr;
   int* d_meta = nullptr;                // per slot: j, rmax, kxw, (int)mmax -- device copy
   int64_t* d_off = nullptr;
-  int64_t* d_mmax = nullptr;
   // x pass fused with the contraction: shells 2p and 2p+1 share FFT line p;
   // one item (p, kx) per nonzero input column kx < kxl of line p
   int nxlines = 0, nxitems = 0;

@@ -154,7 +154,6 @@ int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec) {
 struct ShellDev {
   const int* meta;
   const int64_t* off;
-  const int64_t* mmax;
   int step;
   int Nh;
 };
@@ -191,36 +190,43 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_z(const cplx<T>* __restri
   // shell j is the |k|^2 range (mlo, mhi]
   const int64_t mlo = shell_mmax(j - 1, sd.step), mhi = shell_mmax(j, sd.step);
   constexpr int NL = TileCfg<T>::LINES;  // kx columns per tile (the plan's kt)
-  batched_fill<8, cplx<T>>(
-      NL * N,
-      [&](int idx) {
-        const int n = idx / NL, c = idx - n * NL;
-        const int kx = kx0 + c;
-        cplx<T> v = mk(T(0), T(0));
-        if (kx < Nh) {
-          const int kzs = signed_f(n, N);
-          const int64_t m = (int64_t)kx * kx + (int64_t)kys * kys + (int64_t)kzs * kzs;
-          if (m > mlo && m <= mhi) v = H[((int64_t)n * N + ky) * Nh + kx];
-        }
-        return v;
-      },
-      [&](int idx, cplx<T> v) {
-        const int n = idx / NL, c = idx - n * NL;
-        lds[c * lstride + n] = v;
-      });
+  // thread (row n0, column c) loads rows n0, n0 + RS, ... of kx column c, U
+  // loads in flight; |k|^2 <= 3 (N/2)^2 fits in 32 bits, so the shell range
+  // is tested on kx^2 + ky^2 + kz^2 in int with the bounds clamped to it
+  constexpr int RS = FFT_THREADS / NL, U = 8;
+  const int c = (int)threadIdx.x % NL, n0 = (int)threadIdx.x / NL, kx = kx0 + c;
+  const int r2 = kx * kx + kys * kys;
+  const int lo = (int)min(mlo, (int64_t)1 << 30), hi = (int)min(mhi, (int64_t)1 << 30);
+  const bool colin = kx < Nh;
+  const cplx<T>* src = H + (int64_t)ky * Nh + kx;
+  const int64_t rstride = (int64_t)N * Nh;
+  for (int nb = n0; nb < N; nb += RS * U) {
+    cplx<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = nb + u * RS;
+      const int kzs = signed_f(n, N);
+      const int m = r2 + kzs * kzs;
+      v[u] = mk(T(0), T(0));
+      if (colin && n < N && m > lo && m <= hi) v[u] = src[(int64_t)n * rstride];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = nb + u * RS;
+      if (n < N) lds[c * lstride + n] = v[u];
+    }
+  }
   cplx<T>* twl = lds + NL * lstride;
   stage_twiddles(twl, tw, N);
   __syncthreads();
   fft_tile<NC, -1, T, NL, NL * 1024>(lds, sh, NL, lstride, twl);
-  cplx<T>* dst = T1 + sd.off[s] + (int64_t)ky * kxw + kx0;
-  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
-    const int n = idx / NL, c = idx - n * NL;
-    dst[(int64_t)n * N * kxw + c] = lds[c * lstride + n];
-  }
+  cplx<T>* dst = T1 + sd.off[s] + (int64_t)ky * kxw + kx0 + c;
+  const int64_t dstride = (int64_t)N * kxw;
+  for (int n = n0; n < N; n += RS) dst[(int64_t)n * dstride] = lds[c * lstride + n];
 }
 
 // y pass: tile (s, z, kx0): NL kx columns at one z; rows (ky, kx-tile) that
-// the z pass skipped are zero.
+// the z pass skipped (|ky| > kyl, the tile's c) are zero and not read.
 template <typename T, int NC>
 __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restrict__ T1,
                                                          cplx<T>* __restrict__ T2,
@@ -236,30 +242,34 @@ __global__ __launch_bounds__(FFT_THREADS) void k_shell_y(const cplx<T>* __restri
   const Tile3 t = tiles[blockIdx.x];
   const int s = t.s, z = t.a, kx0 = t.b;
   const int kxw = sd.meta[4 * s + 2];
-  const int64_t mm = sd.mmax[s];
   const int64_t base = sd.off[s] + (int64_t)z * N * kxw + kx0;
   constexpr int NL = TileCfg<T>::LINES;
-  batched_fill<8, cplx<T>>(
-      NL * N,
-      [&](int idx) {
-        const int n = idx / NL, c = idx - n * NL;
-        const int kys = signed_f(n, N);
-        cplx<T> v = mk(T(0), T(0));
-        if ((int64_t)kx0 * kx0 + (int64_t)kys * kys <= mm) v = T1[base + (int64_t)n * kxw + c];
-        return v;
-      },
-      [&](int idx, cplx<T> v) {
-        const int n = idx / NL, c = idx - n * NL;
-        lds[c * lstride + n] = v;
-      });
+  // thread (row n0, column c) loads rows n0, n0 + RS, ... of column c, U loads
+  // in flight; the rows |ky| <= kyl (kx0^2 + ky^2 <= mm) hold z-pass output
+  constexpr int RS = FFT_THREADS / NL, U = 8;
+  const int c = (int)threadIdx.x % NL, n0 = (int)threadIdx.x / NL;
+  const int kyl = t.c;
+  const cplx<T>* src = T1 + base + c;
+  for (int nb = n0; nb < N; nb += RS * U) {
+    cplx<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = nb + u * RS;
+      v[u] = mk(T(0), T(0));
+      if (n < N && (n <= kyl || n >= N - kyl)) v[u] = src[(int64_t)n * kxw];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = nb + u * RS;
+      if (n < N) lds[c * lstride + n] = v[u];
+    }
+  }
   cplx<T>* twl = lds + NL * lstride;
   stage_twiddles(twl, tw, N);
   __syncthreads();
   fft_tile<NC, -1, T, NL, NL * 1024>(lds, sh, NL, lstride, twl);
-  for (int idx = threadIdx.x; idx < NL * N; idx += FFT_THREADS) {
-    const int n = idx / NL, c = idx - n * NL;
-    T2[base + (int64_t)n * kxw + c] = lds[c * lstride + n];
-  }
+  cplx<T>* dst = T2 + base + c;
+  for (int n = n0; n < N; n += RS) dst[(int64_t)n * kxw] = lds[c * lstride + n];
 }
 
 // x pass (Hermitian -> real): tile (s, z, y0) covers rows y0..y0+2NP-1 as NP
@@ -347,7 +357,7 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
                           int nbox = 1) {
   const int64_t hbs = (int64_t)p->shape.N * p->shape.N * (p->shape.N / 2 + 1);
   const int N = p->shape.N;
-  ShellDev sd{sp->d_meta, sp->d_off, sp->d_mmax, step, N / 2 + 1};
+  ShellDev sd{sp->d_meta, sp->d_off, step, N / 2 + 1};
   const size_t ldsz = ((size_t)TileCfg<T>::LINES * (N + 1) + N) * sizeof(cplx<T>);
   const size_t ldsx = ((size_t)shell_x_pairs<T>(NC) * (N + PSX_XPAD) + N) * sizeof(cplx<T>);
   const cplx<T>* tw = plan_tw<T>(p);
