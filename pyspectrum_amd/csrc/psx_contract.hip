@@ -98,7 +98,7 @@ template <> struct CtTraits<PSX_PREC_BF16X3> {
 // 12 waves x 2 tiles (168 VGPRs, so 3 waves per SIMD overlap each other's LDS,
 // VALU and MFMA latencies; measured 4.5 % faster than 8 x 4 at Nmax 40).
 // FP32: 12 waves x 2 tiles (at most 166 VGPRs): 24 pair tiles per group, so
-// at 40 shells (22 tiles) one group reads D once (8 x 2 needed two groups,
+// at 40 shells (21 tiles) one group reads D once (8 x 2 needed two groups,
 // each reading all of D).  Otherwise 8 waves with the mode's tiles per wave
 // (split-BF16 at 4 l tiles: 3, whose float32 and float64 accumulators fit
 // 256 VGPRs at 2 waves per SIMD).
