@@ -317,9 +317,8 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
       __syncthreads();
     }
     constexpr int Q = KC / 4;
-    for (int i = tid; i < (ns + 1) * Q; i += XC_THREADS) {
-      const int s = i / Q, k = (i - s * Q) * 4;
-      const f4v x = *reinterpret_cast<const f4v*>(rows + s * L + k);
+    auto split4 = [&](const float* src, unsigned short* dh, unsigned short* dl) {
+      const f4v x = *reinterpret_cast<const f4v*>(src);
       u2v h, l;
       const unsigned h0 = pack_bf16(x[0], x[1]), h1 = pack_bf16(x[2], x[3]);
       float r0, r1, r2, r3;
@@ -327,10 +326,28 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
       bf16_residuals(h1, x[2], x[3], neg_lo, neg_hi, r2, r3);
       h[0] = h0; h[1] = h1;
       l[0] = pack_bf16(r0, r1); l[1] = pack_bf16(r2, r3);
-      // point k = 32m + 16h + 4g is stored at 32m + 8g + 4h (the K order)
-      const int kp = (k & ~31) + ((k & 12) << 1) + ((k & 16) >> 2);
-      *reinterpret_cast<u2v*>(ldh + s * LDWH + kp) = h;
-      *reinterpret_cast<u2v*>(ldl + s * LDWH + kp) = l;
+      *reinterpret_cast<u2v*>(dh) = h;
+      *reinterpret_cast<u2v*>(dl) = l;
+    };
+    // point k = 32m + 16h + 4g is stored at 32m + 8g + 4h (the K order)
+    auto korder = [](int k) { return (k & ~31) + ((k & 12) << 1) + ((k & 16) >> 2); };
+    if constexpr (XC_THREADS % Q == 0) {
+      // item i = tid + XC_THREADS * p is row s0 + RPP * p, quad c: the same
+      // items as the flat loop below, with the divisions taken once
+      constexpr int RPP = XC_THREADS / Q;
+      const int s0 = tid / Q, k = (tid - s0 * Q) * 4, kp = korder(k);
+      const float* src = rows + s0 * L + k;
+      unsigned short* dh = ldh + s0 * LDWH + kp;
+      unsigned short* dl = ldl + s0 * LDWH + kp;
+      for (int s = s0; s < ns + 1; s += RPP) {
+        split4(src, dh, dl);
+        src += RPP * L; dh += RPP * LDWH; dl += RPP * LDWH;
+      }
+    } else {
+      for (int i = tid; i < (ns + 1) * Q; i += XC_THREADS) {
+        const int s = i / Q, k = (i - s * Q) * 4;
+        split4(rows + s * L + k, ldh + s * LDWH + korder(k), ldl + s * LDWH + korder(k));
+      }
     }
   };
 
