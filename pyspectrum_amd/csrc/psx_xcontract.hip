@@ -82,11 +82,13 @@ static size_t xc_lds_bytes(int ns, int nlines_ct) {
 // Butterflies are numbered across all lines (b = line * N/R + j) and dealt to
 // the threads in order, so the threads of a partly filled last pass sit in as
 // few waves as possible and the other waves skip it.  FIRST: the input
-// entries kxl <= n <= N - kxl of a line are zero (never stored).
+// entries kxl <= n <= N - kxl of a line are zero (never stored); they are
+// read from the zero entry zc (an address select instead of a branch).
 template <int N, int NLINES, int STAGE, int NS, int TW, typename Emit>
 __device__ __forceinline__ void xc_stages(const cplx<float>* in, cplx<float>* out, int nlines,
                                           const cplx<float>* __restrict__ tw,
-                                          const int* __restrict__ kxl_line, Emit& emit, int tid) {
+                                          const int* __restrict__ kxl_line,
+                                          const cplx<float>* zc, Emit& emit, int tid) {
   constexpr FftShape sh = fft_shape_of(N);
   if constexpr (STAGE < sh.nstages) {
     constexpr int R = sh.radix[STAGE];
@@ -102,12 +104,14 @@ __device__ __forceinline__ void xc_stages(const cplx<float>* in, cplx<float>* ou
         const int line = b / NB, j = b - line * NB;
         const cplx<float>* base = in + line * L;
         const int kxl = FIRST ? kxl_line[line] : 0;
+        // the zero band kxl <= n <= N - kxl: n - kxl < zlen as unsigned
+        const unsigned zlen = FIRST ? (unsigned)max(0, N - 2 * kxl + 1) : 0u;
         cplx<float> v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int n = j + r * NB;
           if constexpr (FIRST)
-            v[r] = (n < kxl || n > N - kxl) ? base[n] : mk(0.0f, 0.0f);
+            v[r] = *((unsigned)(n - kxl) < zlen ? zc : base + n);
           else
             v[r] = base[n];
         }
@@ -130,7 +134,7 @@ __device__ __forceinline__ void xc_stages(const cplx<float>* in, cplx<float>* ou
     }
     __syncthreads();
     xc_stages<N, NLINES, STAGE + 1, NS * R, TW + NS * (R - 1)>(out, const_cast<cplx<float>*>(in),
-                                                               nlines, tw, kxl_line, emit, tid);
+                                                               nlines, tw, kxl_line, zc, emit, tid);
   }
 }
 
@@ -198,6 +202,8 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
   for (int i = threadIdx.x; i < 2 * (ns + 1) * LDWH; i += XC_THREADS) ldh[i] = 0;
   for (int i = threadIdx.x; i < N; i += XC_THREADS) twl[i] = tw[i];
   for (int i = threadIdx.x; i < nlines; i += XC_THREADS) kxl_l[i] = line_kxl[i];
+  if (threadIdx.x < 2) s_pad[threadIdx.x] = 0;  // the zero entry of the first FFT stage
+  const cplx<float>* const zc = reinterpret_cast<const cplx<float>*>(s_pad);
   for (int s = threadIdx.x; s < ns; s += XC_THREADS) {
     s_kxw[s] = meta[4 * s + 2];
     s_kxl[s] = min(meta[4 * s + 1] + 1, Nh);
@@ -371,7 +377,7 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
     // recomputes the FFT and split indices per row instead of holding them
     // in registers through the contraction
     const int tid = (int)in_register(threadIdx.x);
-    xc_stages<N, NLINES, 0, 1, 0>(lines, lines_o, nlines, twl, kxl_l, emit, tid);
+    xc_stages<N, NLINES, 0, 1, 0>(lines, lines_o, nlines, twl, kxl_l, zc, emit, tid);
     PSX_XC_TICK(1);
     split_rows(tid);
     __syncthreads();
