@@ -10,8 +10,8 @@ the reference), computed once before the timed region.
 
 Prints one JSON line: boxes/s of the mixed-precision (split-BF16) path, its
 p50/p99 latency, the FP64 path's boxes/s and the ratio, the device time per
-box of each main kernel (HIP events the library records around its launches
-over the timed region), the roofline of the kernel with the most time and a
+box of each main kernel (HIP event pairs the library's dispatches record over
+the timed region), the roofline of the kernel with the most time and a
 CPU baseline (the reference-structured oracle path, timed on a bounded
 sample of the same box on this host, 1 thread and all cores).
 """
@@ -99,16 +99,17 @@ class Box:
                (2 * np.pi) ** 6, self.kf ** 6, L.ptr(self.out7), L.ptr(self.p0k), L.PREC[prec])
 
 
-# Kernels the library brackets with HIP events while kernel timing is on.
+# Kernels whose launches carry HIP start/stop events while kernel timing is on.
 KERNELS = ("k_assign_lines", "k_fft_lines", "k_fft_zcomb", "k_shell_z", "k_shell_y", "k_shell_x",
            "k_contract", "k_xcontract")
 
 
 def time_steps(box, prec, steps, warmup):
     """Wall time and per-step latency of `steps` boxes, and the device time
-    of each main kernel over the same timed region: the library records HIP
-    events around each launch on the stream the kernels run on (the
-    context's stream, bound to torch's current stream by each call)."""
+    of each main kernel over the same timed region: each launch carries a
+    HIP event pair that its dispatch records, on the stream the kernels run
+    on (the context's stream, bound to torch's current stream by each
+    call)."""
     torch = box.torch
     for _ in range(warmup):
         box.run(prec)

@@ -66,7 +66,8 @@ int psx_ctx_release_workspace(psx_ctx* ctx);
 /* Kernel timing for measurement tools (no reference counterpart): while
  * enabled, each launch of the main kernels (k_assign_lines, k_fft_lines,
  * k_fft_zcomb, k_shell_z, k_shell_y, k_shell_x, k_contract, k_xcontract) is
- * bracketed by a HIP event pair on the context's stream.  Enabling or
+ * launched with a start/stop HIP event pair that the dispatch itself records
+ * (hipExtLaunchKernel: no marker packets between kernels).  Enabling or
  * disabling waits for the stream and clears what was recorded. */
 int psx_ctx_kernel_timing(psx_ctx* ctx, int enable);
 /* Number of launches of `kernel` recorded since timing was enabled and their

@@ -696,26 +696,22 @@ static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut
 
 using namespace psx;
 
-psx::TimedLaunch::TimedLaunch(psx_ctx* ctx, const char* name) : ctx_(ctx) {
-  if (!ctx->timing) return;
+bool psx::timing_events(psx_ctx* ctx, const char* name, hipEvent_t* start, hipEvent_t* stop) {
+  if (!ctx->timing) return false;
   if (ctx->kev_used == ctx->kev.size()) {
     psx_ctx::KernelEvents e{name, nullptr, nullptr};
-    if (hipEventCreate(&e.start) != hipSuccess) return;
+    if (hipEventCreate(&e.start) != hipSuccess) return false;
     if (hipEventCreate(&e.stop) != hipSuccess) {
       (void)hipEventDestroy(e.start);
-      return;
+      return false;
     }
     ctx->kev.push_back(e);
   }
-  psx_ctx::KernelEvents& e = ctx->kev[ctx->kev_used];
+  psx_ctx::KernelEvents& e = ctx->kev[ctx->kev_used++];
   e.name = name;
-  if (hipEventRecord(e.start, ctx->stream) != hipSuccess) return;
-  ++ctx->kev_used;
-  stop_ = e.stop;
-}
-
-psx::TimedLaunch::~TimedLaunch() {
-  if (stop_) (void)hipEventRecord(stop_, ctx_->stream);
+  *start = e.start;
+  *stop = e.stop;
+  return true;
 }
 
 // ================================================================= C ABI ===

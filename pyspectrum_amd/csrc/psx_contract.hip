@@ -489,13 +489,10 @@ static int contract_t(psx_ctx* ctx, const ContractPlan* cp, const void* D, int N
   void* part = nullptr;
   PSX_TRY(ctx_buffer(ctx, "contract_partials",
                      (size_t)ngroups * nwg * nslot * 256 * sizeof(typename Tr::TP), &part));
-  {
-    TimedLaunch timed(ctx, "k_contract");
-    hipLaunchKernelGGL((k_contract<MODE, NLT>), dim3((unsigned)nwg, ngroups), dim3(CT_WAVES * 64), lds,
-                       ctx->stream, (const typename Tr::TD*)D, G, cp->ns, cp->d_wave_pt,
-                       cp->d_wave_mask, cp->d_wave_slot, cp->d_pair_s, pts_per_block, nslot,
-                       (typename Tr::TP*)part);
-  }
+  launch_timed(ctx, "k_contract", k_contract<MODE, NLT>, dim3((unsigned)nwg, ngroups),
+               dim3(CT_WAVES * 64), lds, (const typename Tr::TD*)D, G, cp->ns, cp->d_wave_pt,
+               cp->d_wave_mask, cp->d_wave_slot, cp->d_pair_s, pts_per_block, nslot,
+               (typename Tr::TP*)part);
   PSX_CHECK_LAUNCH();
   return launch_contract_reduce(ctx, cp, (const double*)part, nwg, ngroups, braw);
 }

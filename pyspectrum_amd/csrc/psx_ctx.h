@@ -7,6 +7,8 @@
 #include <tuple>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "psx_common.h"
 #include "psx_fft.h"
 
@@ -176,19 +178,22 @@ inline int set_lds(K kernel, size_t bytes) {
   return PSX_OK;
 }
 
-// Brackets one launch with a HIP event pair on the context's stream while
-// kernel timing is on; does nothing otherwise.  `name` must be a literal.
-class TimedLaunch {
- public:
-  TimedLaunch(psx_ctx* ctx, const char* name);
-  ~TimedLaunch();
-  TimedLaunch(const TimedLaunch&) = delete;
-  TimedLaunch& operator=(const TimedLaunch&) = delete;
+// While kernel timing is on: the start/stop event pair for the next timed
+// launch, recorded under `name` (a literal); false otherwise.
+bool timing_events(psx_ctx* ctx, const char* name, hipEvent_t* start, hipEvent_t* stop);
 
- private:
-  psx_ctx* ctx_;
-  hipEvent_t stop_ = nullptr;
-};
+// Launch `kern` on the context's stream.  While kernel timing is on the
+// launch carries its start/stop event pair: hipExtLaunchKernelGGL has the
+// dispatch itself record them, so no marker packets sit between kernels.
+template <typename F, typename... Args>
+inline void launch_timed(psx_ctx* ctx, const char* name, F kern, dim3 grid, dim3 block,
+                         size_t lds, Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (timing_events(ctx, name, &a, &b))
+    hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, ctx->stream, a, b, 0u, args...);
+  else
+    hipLaunchKernelGGL(kern, grid, block, lds, ctx->stream, args...);
+}
 
 // stage launchers (defined in the .hip files)
 int launch_fft3d(psx_ctx* ctx, void* grid, int N, int sign, int prec);

@@ -87,11 +87,8 @@ static int run_pass_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* in, cplx<T>
   const size_t lds = ((size_t)tl * (N + 1) + N) * sizeof(cplx<T>);
   auto kern = k_fft_lines<T, S, NC>;
   PSX_TRY(set_lds(kern, lds));
-  {
-    TimedLaunch timed(ctx, "k_fft_lines");
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks, (unsigned)nbox), dim3(FFT_THREADS), lds,
-                       ctx->stream, in, out, p->shape, plan_tw<T>(p), Os, Cs, Es, N, NN * N);
-  }
+  launch_timed(ctx, "k_fft_lines", kern, dim3((unsigned)nblocks, (unsigned)nbox),
+               dim3(FFT_THREADS), lds, in, out, p->shape, plan_tw<T>(p), Os, Cs, Es, N, NN * N);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }
@@ -363,31 +360,21 @@ static int shell_fields_n(psx_ctx* ctx, const FftPlan* p, const ShellPlan* sp, c
   const cplx<T>* tw = plan_tw<T>(p);
   if (sp->nz_tiles > 0) {
     PSX_TRY(set_lds(k_shell_z<T, NC>, ldsz));
-    {
-      TimedLaunch timed(ctx, "k_shell_z");
-      hipLaunchKernelGGL((k_shell_z<T, NC>), dim3(sp->nz_tiles, nbox), dim3(FFT_THREADS), ldsz,
-                         ctx->stream, H, t1, sp->d_ztiles, sd, p->shape, tw, hbs, sp->t_elems);
-    }
+    launch_timed(ctx, "k_shell_z", k_shell_z<T, NC>, dim3(sp->nz_tiles, nbox), dim3(FFT_THREADS),
+                 ldsz, H, t1, sp->d_ztiles, sd, p->shape, tw, hbs, sp->t_elems);
     PSX_CHECK_LAUNCH();
   }
   if (sp->ny_tiles > 0) {
     PSX_TRY(set_lds(k_shell_y<T, NC>, ldsz));
-    {
-      TimedLaunch timed(ctx, "k_shell_y");
-      hipLaunchKernelGGL((k_shell_y<T, NC>), dim3(sp->ny_tiles, nbox), dim3(FFT_THREADS), ldsz,
-                         ctx->stream, (const cplx<T>*)t1, t2, sp->d_ytiles, sd, p->shape, tw,
-                         sp->t_elems);
-    }
+    launch_timed(ctx, "k_shell_y", k_shell_y<T, NC>, dim3(sp->ny_tiles, nbox), dim3(FFT_THREADS),
+                 ldsz, (const cplx<T>*)t1, t2, sp->d_ytiles, sd, p->shape, tw, sp->t_elems);
     PSX_CHECK_LAUNCH();
   }
   if (do_x && sp->nx_tiles > 0) {
     PSX_TRY(set_lds(k_shell_x<T, NC>, ldsx));
-    {
-      TimedLaunch timed(ctx, "k_shell_x");
-      hipLaunchKernelGGL((k_shell_x<T, NC>), dim3(sp->nx_tiles), dim3(FFT_THREADS), ldsx, ctx->stream,
-                         (const cplx<T>*)t2, dout, sp->d_xtiles, sd, p->shape, tw,
-                         N % 4 == 0 && ((uintptr_t)dout % 16) == 0);
-    }
+    launch_timed(ctx, "k_shell_x", k_shell_x<T, NC>, dim3(sp->nx_tiles), dim3(FFT_THREADS), ldsx,
+                 (const cplx<T>*)t2, dout, sp->d_xtiles, sd, p->shape, tw,
+                 N % 4 == 0 && ((uintptr_t)dout % 16) == 0);
     PSX_CHECK_LAUNCH();
   }
   return PSX_OK;

@@ -392,15 +392,13 @@ static int assign_lines_n(psx_ctx* ctx, const float4* rec, const unsigned* off,
     const size_t lds = acc_bytes > fft_bytes ? acc_bytes : fft_bytes;
     auto kern = k_assign_lines<T, NC, true>;
     PSX_TRY(set_lds(kern, lds));
-    TimedLaunch timed(ctx, "k_assign_lines");
-    hipLaunchKernelGGL(kern, blocks, dim3(FFT_THREADS), lds, ctx->stream, rec, off, bs, p->shape,
-                       at, plan_tw<T>(p), grid);
+    launch_timed(ctx, "k_assign_lines", kern, blocks, dim3(FFT_THREADS), lds, rec, off, bs,
+                 p->shape, at, plan_tw<T>(p), grid);
   } else {
     auto kern = k_assign_lines<T, 0, false>;
     PSX_TRY(set_lds(kern, acc_bytes));
-    TimedLaunch timed(ctx, "k_assign_lines");
-    hipLaunchKernelGGL(kern, blocks, dim3(FFT_THREADS), acc_bytes, ctx->stream, rec, off, bs,
-                       p->shape, at, plan_tw<T>(p), grid);
+    launch_timed(ctx, "k_assign_lines", kern, blocks, dim3(FFT_THREADS), acc_bytes, rec, off, bs,
+                 p->shape, at, plan_tw<T>(p), grid);
   }
   PSX_CHECK_LAUNCH();
   return PSX_OK;
@@ -706,9 +704,8 @@ static int zcomb_n(psx_ctx* ctx, const FftPlan* p, const cplx<T>* F, cplx<T>* Hh
   auto kern = k_fft_zcomb<T, NC>;
   PSX_TRY(set_lds(kern, lds));
   const dim3 grid((unsigned)((int64_t)N * ((Nh + tp - 1) / tp)), (unsigned)nbox);
-  TimedLaunch timed(ctx, "k_fft_zcomb");
-  hipLaunchKernelGGL(kern, grid, dim3(FFT_THREADS), lds, ctx->stream, F, Hh, p->shape,
-                     plan_tw<T>(p), c, (const c64*)p->rec, p->Wf, p->Wd);
+  launch_timed(ctx, "k_fft_zcomb", kern, grid, dim3(FFT_THREADS), lds, F, Hh, p->shape,
+               plan_tw<T>(p), c, (const c64*)p->rec, p->Wf, p->Wd);
   PSX_CHECK_LAUNCH();
   return PSX_OK;
 }

@@ -528,14 +528,10 @@ static int xcontract_t(psx_ctx* ctx, const ContractPlan* cp, const ShellPlan* sp
   PSX_TRY(ctx_buffer(ctx, "xc_phase_clocks", (size_t)nwg * XC_WAVES * 8 * sizeof(long long),
                      (void**)&clocks));
 #endif
-  {
-    TimedLaunch timed(ctx, "k_xcontract");
-    hipLaunchKernelGGL(kern, dim3((unsigned)nwg, (unsigned)nbox), dim3(XC_THREADS), lds,
-                       ctx->stream, t2, sp->d_off, sp->d_meta, ns, sp->d_xitems, sp->nxitems,
-                       sp->d_xline_kxl, plan_tw<float>(fp), cp->d_wave_pt, cp->d_wave_mask,
-                       cp->d_wave_slot, cp->d_pair_s, rows_per_block, cp->nslot, (double*)part,
-                       clocks, sp->t_elems);
-  }
+  launch_timed(ctx, "k_xcontract", kern, dim3((unsigned)nwg, (unsigned)nbox), dim3(XC_THREADS), lds,
+               t2, sp->d_off, sp->d_meta, ns, sp->d_xitems, sp->nxitems, sp->d_xline_kxl,
+               plan_tw<float>(fp), cp->d_wave_pt, cp->d_wave_mask, cp->d_wave_slot, cp->d_pair_s,
+               rows_per_block, cp->nslot, (double*)part, clocks, sp->t_elems);
   PSX_CHECK_LAUNCH();
 #ifdef PSX_XC_PHASE_CLOCKS
   {
