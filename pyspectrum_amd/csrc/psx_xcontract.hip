@@ -184,7 +184,7 @@ __global__ __launch_bounds__(XC_THREADS) void k_xcontract(
   int* kxl_l = reinterpret_cast<int*>(twl + N);
   int* s_kxw = kxl_l + NLINES;
   int* s_kxl = s_kxw + 64;
-  int* s_pad = s_kxl + 64;
+  int* s_pad = s_kxl + 64;  // [0], [1]: the zero complex entry of the first FFT stage
   int64_t* s_off = reinterpret_cast<int64_t*>(s_pad + 64);
   // The out-of-place x pass alternates between the lines over the float rows
   // (A) and lines over the bf16 planes (B), both free while a row is
