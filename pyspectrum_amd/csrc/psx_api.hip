@@ -507,12 +507,60 @@ static int fft_periodic_impl(psx_ctx* ctx, const float* xyz, const float* w, int
   return mesh_to_half(ctx, xyz, w, np, N, pm, sumw, wsum_abs, half, prec);
 }
 
+// The side stream: fork_side orders it after the work queued on the main
+// stream so far; join_side makes the main stream wait for what was queued
+// on it.  Both are stream operations only (a HIP graph capture of the main
+// stream follows the fork and the join).
+static int fork_side(psx_ctx* ctx) {
+  if (!ctx->side) {
+    PSX_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    PSX_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    PSX_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+  }
+  PSX_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+  PSX_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+  return PSX_OK;
+}
+
+static int join_side(psx_ctx* ctx) {
+  PSX_HIP(hipEventRecord(ctx->join_ev, ctx->side));
+  PSX_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
+  return PSX_OK;
+}
+
+static int bk_raw_main(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                       double* braw, int prec, int fp, int ns, int64_t G);
+
 static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                        double* braw, double* p0k, int prec) {
   const int fp = fft_prec(prec);
   const int j0 = ncut / step;
   const int ns = nmax - j0 + 1;
   const int64_t G = (int64_t)N * N * N;
+  if (p0k) {
+    // the shell powers read only the half field: they run on the side
+    // stream beside the shell passes and the contraction (a latency-bound
+    // kernel next to HBM-bound ones) and are joined below
+    void* bins;
+    PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)3 * nmax * sizeof(double), &bins));
+    PSX_TRY(fork_side(ctx));
+    const hipStream_t main_stream = ctx->stream;
+    ctx->stream = ctx->side;
+    int rc = launch_bin_power(ctx, half, N, step, nmax, 1.0, fp, (double*)bins);
+    if (rc == PSX_OK) rc = launch_p0k(ctx, (const double*)bins, nmax, j0, p0k);
+    ctx->stream = main_stream;
+    if (rc != PSX_OK) {
+      (void)join_side(ctx);
+      return rc;
+    }
+  }
+  const int rc = bk_raw_main(ctx, half, N, step, ncut, nmax, braw, prec, fp, ns, G);
+  if (p0k) PSX_TRY(join_side(ctx));
+  return rc;
+}
+
+static int bk_raw_main(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                       double* braw, int prec, int fp, int ns, int64_t G) {
   if (xcontract_supported(ctx, N, step, ncut, nmax, prec) == PSX_OK) {
     // split-BF16 with the shell x pass fused into the contraction: the shell
     // fields never go through HBM
@@ -522,12 +570,6 @@ static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut
     PSX_TRY(ctx_buffer(ctx, "shells", (size_t)ns * G * rbytes(fp), &D));
     PSX_TRY(launch_shell_fields(ctx, half, N, step, ncut, nmax, fp, D));
     PSX_TRY(launch_contract(ctx, D, N, step, ncut, nmax, prec, braw));
-  }
-  if (p0k) {
-    void* bins;
-    PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)3 * nmax * sizeof(double), &bins));
-    PSX_TRY(launch_bin_power(ctx, half, N, step, nmax, 1.0, fp, (double*)bins));
-    PSX_TRY(launch_p0k(ctx, (const double*)bins, nmax, j0, p0k));
   }
   return PSX_OK;
 }
@@ -585,6 +627,9 @@ int psx_ctx_destroy(psx_ctx* ctx) {
   for (auto& kv : ctx->bufs) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (void* p : ctx->owned) (void)hipFree(p);
   if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   for (auto& e : ctx->kev) {
     (void)hipEventDestroy(e.start);
     (void)hipEventDestroy(e.stop);

@@ -130,6 +130,11 @@ struct psx_ctx {
   // streams at once.
   std::recursive_mutex mu;
   hipEvent_t order_ev = nullptr;
+  // Side stream for independent work beside the main sequence (the shell
+  // powers beside the shell passes): forked from and joined back into
+  // `stream` with an event each, within one entry point.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::map<int, psx::FftPlan> fft_plans;
   std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<psx::ShellPlan>> shell_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<psx::ContractPlan>> contract_plans;
