@@ -528,6 +528,24 @@ static int join_side(psx_ctx* ctx) {
   return PSX_OK;
 }
 
+// The shell powers p0k of nbox half fields read only the half fields: they
+// run on the side stream beside the shell passes and the contraction (a
+// latency-bound kernel next to HBM- and issue-bound ones).  The caller joins
+// with join_side before p0k is read (on an error here it is joined already).
+static int p0k_on_side(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
+                       int fp, double* p0k, int nbox) {
+  void* bins;
+  PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)nbox * 3 * nmax * sizeof(double), &bins));
+  PSX_TRY(fork_side(ctx));
+  const hipStream_t main_stream = ctx->stream;
+  ctx->stream = ctx->side;
+  int rc = launch_bin_power(ctx, half, N, step, nmax, 1.0, fp, (double*)bins, nbox);
+  if (rc == PSX_OK) rc = launch_p0k(ctx, (const double*)bins, nmax, ncut / step, p0k, nbox);
+  ctx->stream = main_stream;
+  if (rc != PSX_OK) (void)join_side(ctx);
+  return rc;
+}
+
 static int bk_raw_main(psx_ctx* ctx, const void* half, int N, int step, int ncut, int nmax,
                        double* braw, int prec, int fp, int ns, int64_t G);
 
@@ -537,23 +555,7 @@ static int bk_raw_impl(psx_ctx* ctx, const void* half, int N, int step, int ncut
   const int j0 = ncut / step;
   const int ns = nmax - j0 + 1;
   const int64_t G = (int64_t)N * N * N;
-  if (p0k) {
-    // the shell powers read only the half field: they run on the side
-    // stream beside the shell passes and the contraction (a latency-bound
-    // kernel next to HBM-bound ones) and are joined below
-    void* bins;
-    PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)3 * nmax * sizeof(double), &bins));
-    PSX_TRY(fork_side(ctx));
-    const hipStream_t main_stream = ctx->stream;
-    ctx->stream = ctx->side;
-    int rc = launch_bin_power(ctx, half, N, step, nmax, 1.0, fp, (double*)bins);
-    if (rc == PSX_OK) rc = launch_p0k(ctx, (const double*)bins, nmax, j0, p0k);
-    ctx->stream = main_stream;
-    if (rc != PSX_OK) {
-      (void)join_side(ctx);
-      return rc;
-    }
-  }
+  if (p0k) PSX_TRY(p0k_on_side(ctx, half, N, step, ncut, nmax, fp, p0k, 1));
   const int rc = bk_raw_main(ctx, half, N, step, ncut, nmax, braw, prec, fp, ns, G);
   if (p0k) PSX_TRY(join_side(ctx));
   return rc;
@@ -1054,10 +1056,11 @@ int psx_bk_periodic_batch(psx_ctx* ctx, int nbox, const psx_box* boxes, int ngri
                                    scale.data(), N, grid, fp, true));
   PSX_TRY(launch_fft_axis_batch(ctx, grid, N, 1, fp, 1, nbox));
   PSX_TRY(launch_fft_zcomb_batch(ctx, grid, half, N, nbox, sumw.data(), fp));
-  // shells and contraction
-  if (xcontract_supported(ctx, N, step, ncut, nmax, prec) == PSX_OK) {
-    PSX_TRY(launch_xcontract(ctx, half, N, step, ncut, nmax, (double*)braw, nbox));
-  } else {
+  // shell powers p0k on the side stream; shells and contraction here
+  PSX_TRY(p0k_on_side(ctx, half, N, step, ncut, nmax, fp, p0k, nbox));
+  const int rc = [&]() -> int {
+    if (xcontract_supported(ctx, N, step, ncut, nmax, prec) == PSX_OK)
+      return launch_xcontract(ctx, half, N, step, ncut, nmax, (double*)braw, nbox);
     const int ns = nmax - ncut / step + 1;
     void* D;
     PSX_TRY(ctx_buffer(ctx, "shells", (size_t)ns * G * rbytes(fp), &D));
@@ -1066,12 +1069,11 @@ int psx_bk_periodic_batch(psx_ctx* ctx, int nbox, const psx_box* boxes, int ngri
                                   ncut, nmax, fp, D));
       PSX_TRY(launch_contract(ctx, D, N, step, ncut, nmax, prec, (double*)braw + (size_t)b * ntri));
     }
-  }
-  // shell powers p0k, then units, shot noise and Q
-  void* bins;
-  PSX_TRY(ctx_buffer(ctx, "p0k_bins", (size_t)nbox * 3 * nmax * sizeof(double), &bins));
-  PSX_TRY(launch_bin_power(ctx, half, N, step, nmax, 1.0, fp, (double*)bins, nbox));
-  PSX_TRY(launch_p0k(ctx, (const double*)bins, nmax, ncut / step, p0k, nbox));
+    return PSX_OK;
+  }();
+  PSX_TRY(join_side(ctx));
+  PSX_TRY(rc);
+  // units, shot noise and Q
   if (ntri == 0) return PSX_OK;
   return launch_bk_finalize_batch(ctx, nbox, (const double*)braw, p0k, nmax, counts, ijl, ntri,
                                   (double)G, nbar.data(), a3, b3.data(), a6, b6.data(), out7);
