@@ -142,33 +142,49 @@ __global__ void k_tile_count(const AssignBoxes bs, int N, AssignTiling at,
 }
 
 // Exclusive scan of n counts into off[0..n] (off[n] = total), one workgroup:
-// thread t sums a contiguous segment, the segment sums are scanned in LDS.
-// Box blockIdx.y: count [box][n], off [box][n + 1].
-constexpr int SCAN_THREADS = 1024;
+// thread t owns the contiguous segment [t*seg, (t+1)*seg) and keeps up to
+// SCAN_REG of its counts in registers (one round of loads, no re-read); the
+// segment sums are scanned across each wave with shuffles and the 16 wave
+// totals once in LDS.  Box blockIdx.y: count [box][n], off [box][n + 1].
+constexpr int SCAN_THREADS = 1024, SCAN_WAVES = SCAN_THREADS / 64, SCAN_REG = 32;
 __global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const unsigned* __restrict__ count,
                                                             int n, unsigned* __restrict__ off) {
   count += (int64_t)blockIdx.y * n;
   off += (int64_t)blockIdx.y * (n + 1);
-  __shared__ unsigned part[SCAN_THREADS];
-  const int t = threadIdx.x;
+  __shared__ unsigned wsum[SCAN_WAVES];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int seg = (n + SCAN_THREADS - 1) / SCAN_THREADS;
   const int b = min(n, t * seg), e = min(n, b + seg);
+  unsigned c[SCAN_REG];
   unsigned s = 0;
-  for (int i = b; i < e; ++i) s += count[i];
-  part[t] = s;
+  if (seg <= SCAN_REG) {
+#pragma unroll
+    for (int k = 0; k < SCAN_REG; ++k) c[k] = b + k < e ? count[b + k] : 0u;
+#pragma unroll
+    for (int k = 0; k < SCAN_REG; ++k) s += c[k];
+  } else {
+    for (int i = b; i < e; ++i) s += count[i];
+  }
+  // inclusive scan of the segment sums across the wave, then across waves
+  unsigned inc = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned u = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += u;
+  }
+  if (lane == 63) wsum[wave] = inc;
   __syncthreads();
-  for (int d = 1; d < SCAN_THREADS; d <<= 1) {
-    const unsigned v = t >= d ? part[t - d] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  unsigned before = 0;  // totals of the earlier waves
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  unsigned run = before + inc - s;  // exclusive prefix of this segment
+  if (seg <= SCAN_REG) {
+#pragma unroll
+    for (int k = 0; k < SCAN_REG; ++k)
+      if (b + k < e) { off[b + k] = run; run += c[k]; }
+  } else {
+    for (int i = b; i < e; ++i) { off[i] = run; run += count[i]; }
   }
-  unsigned run = part[t] - s;  // exclusive prefix of this segment
-  for (int i = b; i < e; ++i) {
-    off[i] = run;
-    run += count[i];
-  }
-  if (t == SCAN_THREADS - 1) off[n] = part[t];
+  if (t == SCAN_THREADS - 1) off[n] = before + inc;
 }
 
 __global__ void k_tile_scatter(const AssignBoxes bs, const unsigned* __restrict__ key,
