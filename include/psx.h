@@ -16,7 +16,10 @@
  *    ifft_delta[:N//2+1]) at (z*N + y)*(N/2+1) + x.  Complex values are
  *    interleaved (re, im) of float (prec F32/BF16X3) or double (prec F64).
  *  - Work is enqueued on the context's stream (psx_ctx_set_stream); calls
- *    return before the device finishes unless stated otherwise.
+ *    return before the device finishes unless stated otherwise.  The B(k)
+ *    entry points also use a second stream the context owns, forked from
+ *    and joined back into the context's stream by events within the call,
+ *    so every call stays ordered on the context's stream.
  */
 #ifndef PSX_H
 #define PSX_H
